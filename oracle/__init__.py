@@ -1,0 +1,284 @@
+"""ctypes bindings for the CPU oracle (test infrastructure only).
+
+The oracle is a plain-C restatement of libkfft 0.8.1's double-precision
+transform engine (``oracle/kfft_oracle.c``).  Only ``tests/``, the smoke entry
+point and the CPU-baseline leg of ``bench.py`` load it, as the checker; the
+product library (``kfft_amd``) never calls it.
+
+All arrays are numpy: complex data is ``complex128`` (the same memory layout as
+``ko_cpx{double r, i}``), real data is ``float64``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "build", "libkfft_oracle.so")
+_lib = None
+
+# flag bits, identical to incs/kfft_types.h:46-57
+FLAG_NORMAL = 0
+FLAG_INVERSE = 1 << 0
+FLAG_RENEW = 1 << 1
+FLAG_GENERIC = 1 << 2
+FLAG_GENERIC_ONLY = 1 << 3
+FLAG_EXPAND_SCALAR = 1 << 4
+FLAG_DISABLE_NORM = 1 << 5
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc + OpenMP)."""
+    args = ["make", "-C", _HERE]
+    if force:
+        args.insert(1, "-B")
+    subprocess.run(args, check=True, stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH, mode=C.RTLD_LOCAL)
+        vp, u32, i32, dp = C.c_void_p, C.c_uint32, C.c_int, C.POINTER(C.c_double)
+        sig = {
+            "ko_sqrt_bisect": (C.c_double, [C.c_double]),
+            "ko_sincos": (None, [C.c_double, dp, dp]),
+            "ko_modpow": (u32, [u32, u32, u32]),
+            "ko_prim_root": (u32, [u32]),
+            "ko_prim_root_inv": (u32, [u32, u32]),
+            "ko_next_fast_size": (u32, [u32]),
+            "ko_factor": (i32, [u32, u32, vp, vp, vp]),
+            "ko_plan_cpx": (vp, [u32, u32]),
+            "ko_plan_free": (None, [vp]),
+            "ko_eval_cpx": (i32, [vp, vp, vp]),
+            "ko_eval_cpx_batch": (i32, [vp, vp, vp, u32, i32]),
+            "ko_plan_scalar": (vp, [u32, u32]),
+            "ko_splan_free": (None, [vp]),
+            "ko_eval_scalar": (i32, [vp, vp, vp]),
+            "ko_evali_scalar": (i32, [vp, vp, vp]),
+            "ko_eval_scalar_batch": (i32, [vp, vp, vp, u32, C.c_size_t, i32]),
+            "ko_eval2_cpx": (i32, [u32, u32, u32, vp, vp]),
+            "ko_eval2_scalar": (i32, [u32, u32, u32, vp, vp]),
+            "ko_evali2_scalar": (i32, [u32, u32, u32, vp, vp]),
+            "ko_conv_cpx": (i32, [u32, u32, vp, vp, vp]),
+            "ko_conv_scalar": (i32, [u32, u32, vp, vp, vp]),
+            "ko_conv2_cpx": (i32, [u32, u32, u32, vp, vp, vp]),
+            "ko_conv2_scalar": (i32, [u32, u32, u32, vp, vp, vp]),
+            "ko_dft_ld": (None, [vp, vp, u32, i32]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        L.ko_twiddle.restype = _Cpx
+        L.ko_twiddle.argtypes = [u32, u32, i32]
+        _lib = L
+    return _lib
+
+
+class _Cpx(C.Structure):
+    _fields_ = [("r", C.c_double), ("i", C.c_double)]
+
+
+def _ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+def _check(ret: int, what: str) -> None:
+    if ret != 0:
+        raise RuntimeError(f"oracle {what} returned {ret}")
+
+
+# --- primitives -------------------------------------------------------------
+def sqrt_bisect(x: float) -> float:
+    return lib().ko_sqrt_bisect(float(x))
+
+
+def sincos(x: float):
+    c, s = C.c_double(), C.c_double()
+    lib().ko_sincos(float(x), C.byref(c), C.byref(s))
+    return c.value, s.value
+
+
+def twiddle(i: int, n: int, inverse: bool = False) -> complex:
+    w = lib().ko_twiddle(i, n, int(inverse))
+    return complex(w.r, w.i)
+
+
+def modpow(x, y, m):
+    return lib().ko_modpow(x, y, m)
+
+
+def prim_root(p):
+    return lib().ko_prim_root(p)
+
+
+def prim_root_inv(p, g):
+    return lib().ko_prim_root_inv(p, g)
+
+
+def next_fast_size(n):
+    return lib().ko_next_fast_size(n)
+
+
+def factor(n: int, level: int = 0):
+    """kf_factor: returns (list of (p, m) pairs, list of Rader-registered primes)."""
+    fac = np.zeros(64, np.uint32)
+    primes = np.zeros(32, np.uint32)
+    npr = np.zeros(1, np.uint32)
+    cnt = lib().ko_factor(n, level, _ptr(fac), _ptr(primes), _ptr(npr))
+    pairs = [(int(fac[2 * i]), int(fac[2 * i + 1])) for i in range(cnt)]
+    return pairs, [int(p) for p in primes[: int(npr[0])]]
+
+
+# --- transforms ---------------------------------------------------------------
+class Plan:
+    """A 1D complex oracle plan (kfft_config_cpx semantics, level 0)."""
+
+    def __init__(self, n: int, flags: int = 0):
+        self.n, self.flags = n, flags
+        self._p = lib().ko_plan_cpx(n, flags)
+        if not self._p:
+            raise MemoryError("oracle plan failed")
+
+    def __del__(self):
+        if getattr(self, "_p", None):
+            lib().ko_plan_free(self._p)
+            self._p = None
+
+    def __call__(self, x: np.ndarray, threads: int = 0) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        assert x.shape[-1] == self.n
+        out = np.empty_like(x)
+        batch = x.size // self.n
+        _check(lib().ko_eval_cpx_batch(self._p, _ptr(x), _ptr(out), batch, threads), "eval_cpx")
+        return out
+
+
+def fft(x: np.ndarray, flags: int = 0, threads: int = 0) -> np.ndarray:
+    x = np.asarray(x)
+    return Plan(x.shape[-1], flags)(x, threads)
+
+
+class ScalarPlan:
+    def __init__(self, n: int, flags: int = 0):
+        self.n, self.flags = n, flags
+        self._p = lib().ko_plan_scalar(n, flags)
+        if not self._p:
+            raise MemoryError("oracle scalar plan failed")
+
+    def __del__(self):
+        if getattr(self, "_p", None):
+            lib().ko_splan_free(self._p)
+            self._p = None
+
+    def out_len(self) -> int:
+        if self.n % 2 or (self.flags & FLAG_EXPAND_SCALAR):
+            return self.n
+        return self.n // 2 + 1
+
+    def forward(self, x: np.ndarray, threads: int = 0) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        assert x.shape[-1] == self.n
+        batch = x.size // self.n
+        # full-length buffers: the reference writes up to n entries (EXPAND)
+        out = np.zeros(x.shape[:-1] + (self.n + 1,), np.complex128)
+        _check(lib().ko_eval_scalar_batch(self._p, _ptr(x), _ptr(out), batch, self.n + 1, threads),
+               "eval_scalar")
+        return out[..., : self.out_len()].copy()
+
+    def inverse(self, X: np.ndarray) -> np.ndarray:
+        X = np.asarray(X, dtype=np.complex128)
+        lead = X.shape[:-1]
+        Xf = X.reshape(-1, X.shape[-1])
+        out = np.empty((Xf.shape[0], self.n), np.float64)
+        buf = np.zeros(self.n + 1, np.complex128)
+        for b in range(Xf.shape[0]):
+            buf[:] = 0
+            m = min(Xf.shape[1], self.n + 1)
+            buf[:m] = Xf[b, :m]
+            row = np.empty(self.n, np.float64)
+            _check(lib().ko_evali_scalar(self._p, _ptr(buf), _ptr(row)), "evali_scalar")
+            out[b] = row
+        return out.reshape(lead + (self.n,))
+
+
+def rfft(x, flags: int = 0):
+    x = np.asarray(x)
+    return ScalarPlan(x.shape[-1], flags & ~FLAG_INVERSE).forward(x)
+
+
+def irfft(X, n: int, flags: int = FLAG_INVERSE):
+    return ScalarPlan(n, flags | FLAG_INVERSE).inverse(X)
+
+
+def fft2(a: np.ndarray, flags: int = 0) -> np.ndarray:
+    """2D complex; ``a`` has shape (y, x), x fastest (ext/2d/kfft_cpx2.c)."""
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    y, x = a.shape
+    out = np.empty_like(a)
+    _check(lib().ko_eval2_cpx(x, y, flags, _ptr(a), _ptr(out)), "eval2_cpx")
+    return out
+
+
+def rfft2(a: np.ndarray, flags: int = 0) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    y, x = a.shape
+    out = np.empty((y, x), np.complex128)
+    _check(lib().ko_eval2_scalar(x, y, flags, _ptr(a), _ptr(out)), "eval2_scalar")
+    return out
+
+
+def irfft2(A: np.ndarray, flags: int = FLAG_INVERSE) -> np.ndarray:
+    A = np.ascontiguousarray(A, dtype=np.complex128)
+    y, x = A.shape
+    out = np.empty((y, x), np.float64)
+    _check(lib().ko_evali2_scalar(x, y, flags | FLAG_INVERSE, _ptr(A), _ptr(out)), "evali2_scalar")
+    return out
+
+
+def conv(a, b, flags: int = 0):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    b = np.ascontiguousarray(b, dtype=np.complex128)
+    out = np.empty_like(a)
+    _check(lib().ko_conv_cpx(a.size, flags, _ptr(a), _ptr(b), _ptr(out)), "conv_cpx")
+    return out
+
+
+def conv_scalar(a, b, flags: int = 0):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    out = np.empty_like(a)
+    _check(lib().ko_conv_scalar(a.size, flags, _ptr(a), _ptr(b), _ptr(out)), "conv_scalar")
+    return out
+
+
+def conv2(a, b, flags: int = 0):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    b = np.ascontiguousarray(b, dtype=np.complex128)
+    y, x = a.shape
+    out = np.empty_like(a)
+    _check(lib().ko_conv2_cpx(x, y, flags, _ptr(a), _ptr(b), _ptr(out)), "conv2_cpx")
+    return out
+
+
+def conv2_scalar(a, b, flags: int = 0):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    y, x = a.shape
+    out = np.empty_like(a)
+    _check(lib().ko_conv2_scalar(x, y, flags, _ptr(a), _ptr(b), _ptr(out)), "conv2_scalar")
+    return out
+
+
+def dft_ld(x: np.ndarray, inverse: bool = False) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    out = np.empty_like(x)
+    lib().ko_dft_ld(_ptr(x), _ptr(out), x.size, int(inverse))
+    return out

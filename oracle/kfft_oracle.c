@@ -1,0 +1,1024 @@
+/*
+ * CPU oracle for the kfft transform engine -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Plain-C restatement of libkfft 0.8.1 (double build, reference defaults:
+ * Rader on, RADER_LIMIT=11, PLAN_LEVEL=7, BFLY_LEVEL=5, internal sincos and
+ * bisection sqrt).  Used by tests/ as the parity checker and by bench.py as the
+ * timed CPU baseline.  The product library never links or calls this file.
+ *
+ * Parity pins for each part are listed in the oracle's test module
+ * (tests/test_oracle.py); every function below cites the reference lines it
+ * restates.  Where the reference is undefined (conv2d complex with x != y, see
+ * ext/conv2d/kfft_cpx_conv.c:67) the oracle accepts only the defined case.
+ */
+#include "kfft_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define KO_OK 0
+#define KO_ALLOC_FAIL 1
+#define KO_BUFFER_FAIL 2
+#define KO_IMPROPER_PLAN 4
+#define KO_BADARGUMENTS 5
+
+/* ------------------------------------------------------------------------ */
+/* math primitives                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* Bisection square root, tolerance 1e-3 (math/kfft_custom_math.h:43-67).
+ * kf_factor truncates its result, so its bias decides factorisations. */
+double
+ko_sqrt_bisect(double x) {
+    if (x <= 0)
+        return 0;
+    double lo = 1, hi = 1;
+    if (x < 1)
+        lo = x;
+    else
+        hi = x;
+    while ((hi - lo) > 0.001) {
+        double mid = (lo + hi) / 2;
+        if (mid * mid > x)
+            hi = mid;
+        else
+            lo = mid;
+    }
+    return (lo + hi) / 2;
+}
+
+/* Binary-angle sincos (math/kfft_scd.c:13-103): the phase is divided by 2*pi,
+ * the fractional turn is split greedily into 48 binary digits, and the result
+ * is the product of the unit rotations by 2*pi*2^-k for every set digit.  The
+ * rotation table holds cos/sin(2*pi*2^-k), k = 1..48, computed here in long
+ * double and rounded once. */
+#define KO_ANGLE_BITS 48
+static double ko_rot_c[KO_ANGLE_BITS], ko_rot_s[KO_ANGLE_BITS], ko_frac[KO_ANGLE_BITS];
+static int ko_rot_ready = 0;
+
+static void
+ko_rot_init(void) {
+    if (ko_rot_ready)
+        return;
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    for (int k = 0; k < KO_ANGLE_BITS; k++) {
+        long double a = two_pi * ldexpl(1.0L, -(k + 1));
+        ko_rot_c[k] = (double)cosl(a);
+        ko_rot_s[k] = (double)sinl(a);
+        ko_frac[k] = ldexp(1.0, -(k + 1));
+    }
+    ko_rot_ready = 1;
+}
+
+void
+ko_sincos(double x, double* co, double* si) {
+    ko_rot_init();
+    const double two_pi = 2.0 * 3.141592653589793238462643383279502884197;
+    double t = x / two_pi;
+    int neg = 0;
+    if (t < 0) {
+        neg = 1;
+        t = -t;
+    }
+    *co = 1.0;
+    *si = 0.0;
+    if (t == 0.0)
+        return;
+    if (t > (double)0xffffffffu)
+        return; /* out of range: the reference returns cos=1, sin=0 */
+    t -= (double)(unsigned long)t; /* drop whole turns (int_fract, :33-34) */
+    for (int k = 0; k < KO_ANGLE_BITS; k++) {
+        double rest = t - ko_frac[k];
+        if (rest >= 0.0) {
+            t = rest;
+            double c = *co, s = *si;
+            *co = c * ko_rot_c[k] - s * ko_rot_s[k];
+            *si = c * ko_rot_s[k] + s * ko_rot_c[k];
+        }
+    }
+    if (neg)
+        *si = -*si;
+}
+
+/* kfft_kernel_twiddle (incs/kfft_math.h:206-216): exp(-+2*pi*i*k/n). */
+ko_cpx
+ko_twiddle(uint32_t i, uint32_t n, int inverse) {
+    const double pi = 3.141592653589793238462643383279502884197;
+    double phase = -2 * pi * i / n;
+    if (inverse)
+        phase = -phase;
+    ko_cpx w;
+    ko_sincos(phase, &w.r, &w.i);
+    return w;
+}
+
+/* Recursive square-and-multiply modpow (incs/kfft_math.h:172-180).  The final
+ * multiply is done in 32 bits as in the reference; all operands stay < 2^16
+ * for the primes the engine meets, so it cannot wrap there. */
+uint32_t
+ko_modpow(uint32_t x, uint32_t y, uint32_t m) {
+    if (y == 0)
+        return 1;
+    uint64_t h = ko_modpow(x, y / 2, m) % m;
+    h = (h * h) % m;
+    return (y % 2 == 0) ? (uint32_t)h : (uint32_t)((x * h) % m);
+}
+
+static uint32_t
+ko_gcd(uint32_t a, uint32_t b) {
+    while (b) {
+        uint32_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+/* Smallest primitive root modulo p (kfft_math.c:5-33), trusting p prime. */
+uint32_t
+ko_prim_root(uint32_t p) {
+    uint32_t phi = p - 1, rem = phi, divs[32], nd = 0;
+    for (uint32_t d = 2; d * d <= rem; d++) {
+        if (rem % d == 0) {
+            divs[nd++] = d;
+            while (rem % d == 0)
+                rem /= d;
+        }
+    }
+    if (rem > 1)
+        divs[nd++] = rem;
+    for (uint32_t g = 2; g <= p; g++) {
+        int good = 1;
+        for (uint32_t j = 0; j < nd && good; j++)
+            if (ko_modpow(g, phi / divs[j], p) == 1)
+                good = 0;
+        if (good)
+            return g;
+    }
+    return 0;
+}
+
+/* Multiplicative inverse of g modulo prime p via Fermat (kfft_math.c:35-38). */
+uint32_t
+ko_prim_root_inv(uint32_t p, uint32_t g) {
+    return (ko_gcd(p, g) != 1) ? 0 : ko_modpow(g, p - 2, p);
+}
+
+/* kfft_next_fast_size (kfft_ext.c:38-52): next 2^a 3^b 5^c >= n. */
+uint32_t
+ko_next_fast_size(uint32_t n) {
+    for (;; n++) {
+        uint32_t m = n;
+        while (m % 2 == 0)
+            m /= 2;
+        while (m % 3 == 0)
+            m /= 3;
+        while (m % 5 == 0)
+            m /= 5;
+        if (m <= 1)
+            return n;
+    }
+}
+
+/* kf_factor (kfft_cpx.c:103-143): radix 4 first, then 2, then odd trial
+ * divisors 3,5,7,... while p <= floor(bisect_sqrt(n)); the remainder is taken
+ * whole.  A factor p with p > BFLY_LEVEL and p >= RADER_LIMIT is registered
+ * for Rader while level < PLAN_LEVEL (:131-135). */
+int
+ko_factor(uint32_t n, uint32_t level, uint32_t* fac, uint32_t* primes, uint32_t* nprimes) {
+    uint32_t p = 4, rem = n;
+    uint32_t lim = (uint32_t)ko_sqrt_bisect((double)n);
+    int count = 0;
+    *nprimes = 0;
+    do {
+        while (rem % p) {
+            if (p == 4)
+                p = 2;
+            else if (p == 2)
+                p = 3;
+            else
+                p += 2;
+            if (p > lim)
+                p = rem;
+        }
+        if (level < KO_PLAN_LEVEL && p > KO_BFLY_LEVEL && p >= KO_RADER_LIMIT)
+            primes[(*nprimes)++] = p;
+        rem /= p;
+        fac[2 * count] = p;
+        fac[2 * count + 1] = rem;
+        count++;
+    } while (rem > 1);
+    return count;
+}
+
+/* ------------------------------------------------------------------------ */
+/* 1D complex plan                                                          */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    uint32_t prime, g, ginv;
+    uint32_t *qidx, *pidx; /* g^k and g^-k modulo prime, k < prime-1 */
+    ko_cpx* bhat;          /* FFT_{p-1} of W_p^{g^-k} (kfft_cpx.c:194-206) */
+    ko_plan *fwd, *inv;    /* level+1 sub-plans of length prime-1 */
+} ko_rader;
+
+struct ko_plan {
+    uint32_t n, flags, level;
+    int nfac;
+    uint32_t fac[64];
+    uint32_t nrader;
+    ko_rader rader[32];
+    ko_cpx* tw; /* tw[k] = W_n^k, direction per flags (kfft_cpx.c:155-157) */
+};
+
+static ko_plan* ko_plan_level(uint32_t n, uint32_t flags, uint32_t level);
+
+void
+ko_plan_free(ko_plan* P) {
+    if (!P)
+        return;
+    for (uint32_t i = 0; i < P->nrader; i++) {
+        ko_rader* r = &P->rader[i];
+        free(r->qidx);
+        free(r->pidx);
+        free(r->bhat);
+        ko_plan_free(r->fwd);
+        ko_plan_free(r->inv);
+    }
+    free(P->tw);
+    free(P);
+}
+
+/* Stage recombinations.  Each follows the arithmetic order of the matching
+ * kf_bflyN so that rounding stays close to the reference. */
+
+/* radix 2 (kfft_bfly.c:1-16) */
+static void
+ko_bfly2(ko_cpx* F, uint32_t fstride, const ko_plan* P, uint32_t m) {
+    ko_cpx* G = F + m;
+    for (uint32_t k = 0; k < m; k++) {
+        ko_cpx w = P->tw[k * fstride], t;
+        t.r = G[k].r * w.r - G[k].i * w.i;
+        t.i = G[k].r * w.i + G[k].i * w.r;
+        G[k].r = F[k].r - t.r;
+        G[k].i = F[k].i - t.i;
+        F[k].r += t.r;
+        F[k].i += t.i;
+    }
+}
+
+static inline ko_cpx
+ko_mul(ko_cpx a, ko_cpx b) {
+    ko_cpx c;
+    c.r = a.r * b.r - a.i * b.i;
+    c.i = a.r * b.i + a.i * b.r;
+    return c;
+}
+
+/* radix 4 (kfft_bfly.c:18-57); the +-i rotation flips with the direction */
+static void
+ko_bfly4(ko_cpx* F, uint32_t fstride, const ko_plan* P, uint32_t m) {
+    const int inv = (P->flags & KO_FLAG_INVERSE) != 0;
+    for (uint32_t k = 0; k < m; k++) {
+        ko_cpx* f = F + k;
+        ko_cpx a1 = ko_mul(f[m], P->tw[k * fstride]);
+        ko_cpx a2 = ko_mul(f[2 * m], P->tw[2 * k * fstride]);
+        ko_cpx a3 = ko_mul(f[3 * m], P->tw[3 * k * fstride]);
+        ko_cpx d02 = {f[0].r - a2.r, f[0].i - a2.i};
+        f[0].r += a2.r;
+        f[0].i += a2.i;
+        ko_cpx s13 = {a1.r + a3.r, a1.i + a3.i};
+        ko_cpx d13 = {a1.r - a3.r, a1.i - a3.i};
+        f[2 * m].r = f[0].r - s13.r;
+        f[2 * m].i = f[0].i - s13.i;
+        f[0].r += s13.r;
+        f[0].i += s13.i;
+        if (inv) {
+            f[m].r = d02.r - d13.i;
+            f[m].i = d02.i + d13.r;
+            f[3 * m].r = d02.r + d13.i;
+            f[3 * m].i = d02.i - d13.r;
+        } else {
+            f[m].r = d02.r + d13.i;
+            f[m].i = d02.i - d13.r;
+            f[3 * m].r = d02.r - d13.i;
+            f[3 * m].i = d02.i + d13.r;
+        }
+    }
+}
+
+/* radix 3 (kfft_bfly.c:59-95): half-sum form with sin(2pi/3) from the table */
+static void
+ko_bfly3(ko_cpx* F, uint32_t fstride, const ko_plan* P, uint32_t m) {
+    const double s3 = P->tw[fstride * m].i;
+    for (uint32_t k = 0; k < m; k++) {
+        ko_cpx* f = F + k;
+        ko_cpx a1 = ko_mul(f[m], P->tw[k * fstride]);
+        ko_cpx a2 = ko_mul(f[2 * m], P->tw[2 * k * fstride]);
+        ko_cpx s = {a1.r + a2.r, a1.i + a2.i};
+        ko_cpx d = {a1.r - a2.r, a1.i - a2.i};
+        f[m].r = f[0].r - s.r * .5;
+        f[m].i = f[0].i - s.i * .5;
+        d.r *= s3;
+        d.i *= s3;
+        f[0].r += s.r;
+        f[0].i += s.i;
+        f[2 * m].r = f[m].r + d.i;
+        f[2 * m].i = f[m].i - d.r;
+        f[m].r -= d.i;
+        f[m].i += d.r;
+    }
+}
+
+/* radix 5 (kfft_bfly.c:97-152) with ya = W^(n/5), yb = W^(2n/5) */
+static void
+ko_bfly5(ko_cpx* F, uint32_t fstride, const ko_plan* P, uint32_t m) {
+    const ko_cpx ya = P->tw[fstride * m], yb = P->tw[2 * fstride * m];
+    for (uint32_t u = 0; u < m; u++) {
+        ko_cpx* f = F + u;
+        ko_cpx x0 = f[0];
+        ko_cpx a1 = ko_mul(f[m], P->tw[u * fstride]);
+        ko_cpx a2 = ko_mul(f[2 * m], P->tw[2 * u * fstride]);
+        ko_cpx a3 = ko_mul(f[3 * m], P->tw[3 * u * fstride]);
+        ko_cpx a4 = ko_mul(f[4 * m], P->tw[4 * u * fstride]);
+        ko_cpx s14 = {a1.r + a4.r, a1.i + a4.i}, d14 = {a1.r - a4.r, a1.i - a4.i};
+        ko_cpx s23 = {a2.r + a3.r, a2.i + a3.i}, d23 = {a2.r - a3.r, a2.i - a3.i};
+        f[0].r += s14.r + s23.r;
+        f[0].i += s14.i + s23.i;
+        ko_cpx p5, p6, p11, p12;
+        p5.r = x0.r + s14.r * ya.r + s23.r * yb.r;
+        p5.i = x0.i + s14.i * ya.r + s23.i * yb.r;
+        p6.r = d14.i * ya.i + d23.i * yb.i;
+        p6.i = -d14.r * ya.i - d23.r * yb.i;
+        f[m].r = p5.r - p6.r;
+        f[m].i = p5.i - p6.i;
+        f[4 * m].r = p5.r + p6.r;
+        f[4 * m].i = p5.i + p6.i;
+        p11.r = x0.r + s14.r * yb.r + s23.r * ya.r;
+        p11.i = x0.i + s14.i * yb.r + s23.i * ya.r;
+        p12.r = -d14.i * yb.i + d23.i * ya.i;
+        p12.i = d14.r * yb.i - d23.r * ya.i;
+        f[2 * m].r = p11.r + p12.r;
+        f[2 * m].i = p11.i + p12.i;
+        f[3 * m].r = p11.r - p12.r;
+        f[3 * m].i = p11.i - p12.i;
+    }
+}
+
+/* Direct O(p^2) DFT of the p values F[u], F[u+m], ... (kfft_generic.c:9-36)
+ * with the running twiddle index reduced by one subtraction per step. */
+static void
+ko_generic_direct(ko_cpx* F, ko_cpx* tmp, size_t fstride, const ko_plan* P, uint32_t u,
+                  uint32_t m, uint32_t p) {
+    for (uint32_t q = 0; q < p; q++)
+        tmp[q] = F[u + q * m];
+    for (uint32_t q1 = 0; q1 < p; q1++) {
+        uint32_t k = u + q1 * m;
+        size_t tw = 0;
+        ko_cpx acc = tmp[0];
+        for (uint32_t q = 1; q < p; q++) {
+            tw += fstride * k;
+            if (tw >= P->n)
+                tw -= P->n;
+            ko_cpx t = ko_mul(tmp[q], P->tw[tw]);
+            acc.r += t.r;
+            acc.i += t.i;
+        }
+        F[k] = acc;
+    }
+}
+
+static int ko_eval_level(const ko_plan* P, const ko_cpx* in, ko_cpx* out);
+
+/* Rader prime-length DFT (kfft_rader.c:17-63, convolution :1-15):
+ * gather x[g^k], cyclic convolution of length p-1 with bhat through the
+ * forward sub-plan, pointwise product and the normalising inverse sub-plan,
+ * then scatter to X[g^-k] adding x[0]; X[0] = x[0] + sum_{j>=1} x[j]. */
+static int
+ko_rader_eval(ko_cpx* F, ko_cpx* tmp, const ko_plan* P, uint32_t u, uint32_t m, uint32_t p) {
+    const ko_rader* R = NULL;
+    for (uint32_t i = 0; i < P->nrader; i++)
+        if (P->rader[i].prime == p) {
+            R = &P->rader[i];
+            break;
+        }
+    if (!R)
+        return KO_BADARGUMENTS;
+    ko_cpx x0 = F[u];
+    tmp[0].r = tmp[0].i = 0;
+    uint32_t k = u;
+    for (uint32_t q = 1; q < p; q++) {
+        k += m;
+        tmp[q] = F[R->qidx[q - 1]];
+        tmp[0].r += F[k].r;
+        tmp[0].i += F[k].i;
+    }
+    uint32_t L = p - 1;
+    ko_cpx* conv = (ko_cpx*)malloc(sizeof(ko_cpx) * L);
+    if (!conv)
+        return KO_BUFFER_FAIL;
+    int ret = ko_eval_level(R->fwd, tmp + 1, conv);
+    if (ret == KO_OK) {
+        for (uint32_t i = 0; i < L; i++)
+            conv[i] = ko_mul(conv[i], R->bhat[i]);
+        ret = ko_eval_level(R->inv, conv, tmp + 1);
+    }
+    free(conv);
+    if (ret != KO_OK)
+        return ret;
+    F[u].r += tmp[0].r;
+    F[u].i += tmp[0].i;
+    for (uint32_t q = 1; q < p; q++) {
+        ko_cpx v = {tmp[q].r + x0.r, tmp[q].i + x0.i};
+        F[u + m * R->pidx[q - 1]] = v;
+    }
+    return KO_OK;
+}
+
+/* kf_bfly_generic (kfft_generic.c:38-80): Rader only for the final factor
+ * (m == 1), p >= RADER_LIMIT, no GENERIC flags and primes registered. */
+static int
+ko_bfly_generic(ko_cpx* F, size_t fstride, const ko_plan* P, uint32_t m, uint32_t p) {
+    ko_cpx* tmp = (ko_cpx*)calloc(p, sizeof(ko_cpx));
+    if (!tmp)
+        return KO_BUFFER_FAIL;
+    int ret = KO_OK;
+    const int rader_ok = (m == 1) && (p >= KO_RADER_LIMIT) &&
+                         !(P->flags & (KO_FLAG_GENERIC | KO_FLAG_GENERIC_ONLY)) && P->nrader;
+    for (uint32_t u = 0; u < m && ret == KO_OK; u++) {
+        if (rader_ok)
+            ret = ko_rader_eval(F, tmp, P, u, m, p);
+        else
+            ko_generic_direct(F, tmp, fstride, P, u, m, p);
+    }
+    free(tmp);
+    return ret;
+}
+
+/* kf_work (kfft_work.c:1-59): decimation in time.  The p sub-transforms of
+ * length m read the input decimated by fstride*p; a leaf (m == 1) gathers p
+ * strided samples; then the stage recombines by radix. */
+static int
+ko_work(ko_cpx* out, const ko_cpx* in, uint32_t fstride, const uint32_t* fac, const ko_plan* P) {
+    if (P->flags & KO_FLAG_GENERIC_ONLY)
+        return ko_bfly_generic(out, 1, P, 1, P->n);
+    const uint32_t p = fac[0], m = fac[1];
+    if (m == 1) {
+        for (uint32_t j = 0; j < p; j++)
+            out[j] = in[(size_t)j * fstride];
+    } else {
+        for (uint32_t q = 0; q < p; q++) {
+            int r = ko_work(out + (size_t)q * m, in + (size_t)q * fstride, fstride * p, fac + 2, P);
+            if (r != KO_OK)
+                return r;
+        }
+    }
+    switch (p) {
+    case 2:
+        ko_bfly2(out, fstride, P, m);
+        return KO_OK;
+    case 3:
+        ko_bfly3(out, fstride, P, m);
+        return KO_OK;
+    case 4:
+        ko_bfly4(out, fstride, P, m);
+        return KO_OK;
+    case 5:
+        ko_bfly5(out, fstride, P, m);
+        return KO_OK;
+    default:
+        return ko_bfly_generic(out, fstride, P, m, p);
+    }
+}
+
+/* kfft_eval_cpx (kfft_cpx.c:320-366).  Normalisation: a level-0 inverse
+ * divides by n unless DISABLE_NORM; nested inverse plans always divide. */
+static int
+ko_eval_level(const ko_plan* P, const ko_cpx* in, ko_cpx* out) {
+    int ret;
+    if (P->flags & KO_FLAG_GENERIC_ONLY) {
+        if (in != out)
+            memcpy(out, in, sizeof(ko_cpx) * P->n);
+        ret = ko_work(out, NULL, 1, NULL, P);
+    } else if (in == out) {
+        ko_cpx* tmp = (ko_cpx*)calloc(P->n, sizeof(ko_cpx));
+        if (!tmp)
+            return KO_BUFFER_FAIL;
+        ret = ko_work(tmp, in, 1, P->fac, P);
+        if (ret == KO_OK)
+            memcpy(out, tmp, sizeof(ko_cpx) * P->n);
+        free(tmp);
+    } else {
+        ret = ko_work(out, in, 1, P->fac, P);
+    }
+    if (ret == KO_OK && (P->flags & KO_FLAG_INVERSE)) {
+        if (P->level > 0 || !(P->flags & KO_FLAG_DISABLE_NORM)) {
+            for (uint32_t i = 0; i < P->n; i++) {
+                out[i].r /= P->n;
+                out[i].i /= P->n;
+            }
+        }
+    }
+    return ret;
+}
+
+/* kfft_config_lvlcpx + kfft_kinit (kfft_cpx.c:145-218, :266-313). */
+static ko_plan*
+ko_plan_level(uint32_t n, uint32_t flags, uint32_t level) {
+    if (n == 0)
+        return NULL;
+    ko_plan* P = (ko_plan*)calloc(1, sizeof(ko_plan));
+    if (!P)
+        return NULL;
+    P->n = n;
+    P->flags = flags;
+    P->level = level;
+    uint32_t primes[32], np = 0;
+    P->nfac = ko_factor(n, level, P->fac, primes, &np);
+    P->tw = (ko_cpx*)malloc(sizeof(ko_cpx) * n);
+    if (!P->tw) {
+        free(P);
+        return NULL;
+    }
+    const int inv = (flags & KO_FLAG_INVERSE) != 0;
+    for (uint32_t i = 0; i < n; i++)
+        P->tw[i] = ko_twiddle(i, n, inv);
+    if (np == 0 || (flags & KO_FLAG_GENERIC_ONLY))
+        return P;
+    for (uint32_t j = 0; j < np; j++) {
+        ko_rader* R = &P->rader[P->nrader++];
+        const uint32_t p = primes[j], L = p - 1;
+        R->prime = p;
+        R->g = ko_prim_root(p);
+        R->ginv = ko_prim_root_inv(p, R->g);
+        R->qidx = (uint32_t*)malloc(sizeof(uint32_t) * L);
+        R->pidx = (uint32_t*)malloc(sizeof(uint32_t) * L);
+        R->bhat = (ko_cpx*)malloc(sizeof(ko_cpx) * L);
+        const uint32_t sub_flags = flags & ~(uint32_t)KO_FLAG_RENEW;
+        R->fwd = ko_plan_level(L, sub_flags & ~(uint32_t)KO_FLAG_INVERSE, level + 1);
+        R->inv = ko_plan_level(L, sub_flags | KO_FLAG_INVERSE, level + 1);
+        if (!R->qidx || !R->pidx || !R->bhat || !R->fwd || !R->inv) {
+            ko_plan_free(P);
+            return NULL;
+        }
+        for (uint32_t k = 0; k < L; k++) {
+            R->qidx[k] = ko_modpow(R->g, k, p);
+            R->pidx[k] = ko_modpow(R->ginv, k, p);
+        }
+        for (uint32_t k = 0; k < L; k++)
+            R->bhat[k] = ko_twiddle(R->pidx[k], p, inv);
+        if (ko_eval_level(R->fwd, R->bhat, R->bhat) != KO_OK) {
+            ko_plan_free(P);
+            return NULL;
+        }
+    }
+    return P;
+}
+
+ko_plan*
+ko_plan_cpx(uint32_t n, uint32_t flags) {
+    return ko_plan_level(n, flags & ~(uint32_t)KO_FLAG_RENEW, 0);
+}
+
+int
+ko_eval_cpx(const ko_plan* P, const ko_cpx* in, ko_cpx* out) {
+    return ko_eval_level(P, in, out);
+}
+
+int
+ko_eval_cpx_batch(const ko_plan* P, const ko_cpx* in, ko_cpx* out, uint32_t batch, int threads) {
+    int bad = 0;
+#ifdef _OPENMP
+    if (threads <= 0)
+        threads = omp_get_max_threads();
+#pragma omp parallel for schedule(static) num_threads(threads) reduction(| : bad)
+#endif
+    for (int64_t b = 0; b < (int64_t)batch; b++)
+        bad |= (ko_eval_level(P, in + (size_t)b * P->n, out + (size_t)b * P->n) != KO_OK);
+    (void)threads;
+    return bad ? KO_BUFFER_FAIL : KO_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* 1D real input ("scalar")                                                  */
+/* ------------------------------------------------------------------------ */
+
+struct ko_splan {
+    uint32_t n, flags;
+    ko_plan* basis; /* length n (odd) or n/2 (even), level 0 */
+    ko_cpx* st;     /* super twiddles for even n (kfft_scalar.c:9-18) */
+};
+
+void
+ko_splan_free(ko_splan* S) {
+    if (!S)
+        return;
+    ko_plan_free(S->basis);
+    free(S->st);
+    free(S);
+}
+
+/* kfft_config_scalar (kfft_scalar.c:317-321): odd n -> complex plan of n
+ * (config_trivial :54-76); even n -> complex plan of n/2 plus super twiddles
+ * st[i] = exp(-+i*pi*((i+1)/(n/2) + 1/2)) (config_nyquist :136-172). */
+ko_splan*
+ko_plan_scalar(uint32_t n, uint32_t flags) {
+    if (n == 0)
+        return NULL;
+    ko_splan* S = (ko_splan*)calloc(1, sizeof(ko_splan));
+    if (!S)
+        return NULL;
+    S->n = n;
+    S->flags = flags;
+    const uint32_t cflags = flags & ~(uint32_t)KO_FLAG_RENEW;
+    if (n % 2) {
+        S->basis = ko_plan_level(n, cflags, 0);
+    } else {
+        const uint32_t h = n / 2;
+        S->basis = ko_plan_level(h, cflags, 0);
+        S->st = (ko_cpx*)malloc(sizeof(ko_cpx) * h);
+        if (S->st) {
+            const double pi = 3.141592653589793238462643383279502884197;
+            for (uint32_t i = 0; i < h; i++) {
+                double phase = -pi * ((double)(i + 1) / h + .5);
+                if (flags & KO_FLAG_INVERSE)
+                    phase = -phase;
+                ko_sincos(phase, &S->st[i].r, &S->st[i].i);
+            }
+        }
+    }
+    if (!S->basis || (n % 2 == 0 && !S->st)) {
+        ko_splan_free(S);
+        return NULL;
+    }
+    return S;
+}
+
+/* rebuild_forward (kfft_scalar.c:174-216) */
+static void
+ko_rebuild_forward(const ko_splan* S, const ko_cpx* Z, ko_cpx* X) {
+    const uint32_t n = S->n, h = S->basis->n;
+    const ko_cpx dc = Z[0];
+    X[0].r = dc.r + dc.i;
+    X[h].r = dc.r - dc.i;
+    X[h].i = X[0].i = 0;
+    for (uint32_t k = 1; k <= h / 2; k++) {
+        ko_cpx a = Z[k];
+        ko_cpx b = {Z[h - k].r, -Z[h - k].i};
+        ko_cpx s = {a.r + b.r, a.i + b.i};
+        ko_cpx d = {a.r - b.r, a.i - b.i};
+        ko_cpx t = ko_mul(d, S->st[k - 1]);
+        X[k].r = (s.r + t.r) * .5;
+        X[k].i = (s.i + t.i) * .5;
+        X[h - k].r = (s.r - t.r) * .5;
+        X[h - k].i = (t.i - s.i) * .5;
+    }
+    if (S->flags & KO_FLAG_EXPAND_SCALAR) {
+        for (uint32_t k = 1; k < n; k++) {
+            X[n - k].r = X[k].r;
+            X[n - k].i = -X[k].i;
+        }
+    }
+}
+
+/* rebuild_inverse (kfft_scalar.c:218-242): produces 2*Z (no 1/2 factor). */
+static void
+ko_rebuild_inverse(const ko_splan* S, const ko_cpx* X, ko_cpx* Z) {
+    const uint32_t h = S->basis->n;
+    Z[0].r = X[0].r + X[h].r;
+    Z[0].i = X[0].r - X[h].r;
+    for (uint32_t k = 1; k <= h / 2; k++) {
+        ko_cpx a = X[k];
+        ko_cpx b = {X[h - k].r, -X[h - k].i};
+        ko_cpx e = {a.r + b.r, a.i + b.i};
+        ko_cpx d = {a.r - b.r, a.i - b.i};
+        ko_cpx o = ko_mul(d, S->st[k - 1]);
+        Z[k].r = e.r + o.r;
+        Z[k].i = e.i + o.i;
+        Z[h - k].r = e.r - o.r;
+        Z[h - k].i = e.i - o.i;
+        Z[h - k].i *= -1;
+    }
+}
+
+/* kfft_eval_scalar_internal (kfft_scalar.c:295-304) with eval_trivial (:78-98)
+ * and eval_nyquist (:244-266). */
+int
+ko_eval_scalar(const ko_splan* S, const double* in, ko_cpx* out) {
+    if (S->basis->flags & KO_FLAG_INVERSE)
+        return KO_IMPROPER_PLAN;
+    int ret;
+    if (S->n % 2) {
+        ko_cpx* buf = (ko_cpx*)calloc(S->n, sizeof(ko_cpx));
+        if (!buf)
+            return KO_BUFFER_FAIL;
+        for (uint32_t i = 0; i < S->n; i++)
+            buf[i].r = in[i];
+        ret = ko_eval_level(S->basis, buf, out);
+        free(buf);
+    } else {
+        const uint32_t h = S->basis->n;
+        ko_cpx* buf = (ko_cpx*)calloc(h + 1, sizeof(ko_cpx));
+        ko_cpx* zin = (ko_cpx*)malloc(sizeof(ko_cpx) * h);
+        if (!buf || !zin) {
+            free(buf);
+            free(zin);
+            return KO_BUFFER_FAIL;
+        }
+        memcpy(zin, in, sizeof(ko_cpx) * h); /* reals read as n/2 complex (:254) */
+        ret = ko_eval_level(S->basis, zin, buf);
+        if (ret == KO_OK)
+            ko_rebuild_forward(S, buf, out);
+        free(buf);
+        free(zin);
+    }
+    return ret;
+}
+
+/* kfft_evali_scalar_internal (kfft_scalar.c:306-315) with evali_trivial
+ * (:100-118) and evali_nyquist (:268-289). */
+int
+ko_evali_scalar(const ko_splan* S, const ko_cpx* in, double* out) {
+    if (!(S->basis->flags & KO_FLAG_INVERSE))
+        return KO_IMPROPER_PLAN;
+    int ret;
+    if (S->n % 2) {
+        ko_cpx* buf = (ko_cpx*)calloc(S->n, sizeof(ko_cpx));
+        if (!buf)
+            return KO_BUFFER_FAIL;
+        ret = ko_eval_level(S->basis, in, buf);
+        for (uint32_t i = 0; i < S->n; i++)
+            out[i] = buf[i].r;
+        free(buf);
+    } else {
+        const uint32_t h = S->basis->n;
+        ko_cpx* buf = (ko_cpx*)calloc(h + 1, sizeof(ko_cpx));
+        ko_cpx* zout = (ko_cpx*)malloc(sizeof(ko_cpx) * h);
+        if (!buf || !zout) {
+            free(buf);
+            free(zout);
+            return KO_BUFFER_FAIL;
+        }
+        ko_rebuild_inverse(S, in, buf);
+        ret = ko_eval_level(S->basis, buf, zout);
+        memcpy(out, zout, sizeof(ko_cpx) * h); /* n/2 complex read as n reals (:280) */
+        free(buf);
+        free(zout);
+    }
+    return ret;
+}
+
+int
+ko_eval_scalar_batch(const ko_splan* S, const double* in, ko_cpx* out, uint32_t batch,
+                     size_t out_dist, int threads) {
+    int bad = 0;
+#ifdef _OPENMP
+    if (threads <= 0)
+        threads = omp_get_max_threads();
+#pragma omp parallel for schedule(static) num_threads(threads) reduction(| : bad)
+#endif
+    for (int64_t b = 0; b < (int64_t)batch; b++)
+        bad |= (ko_eval_scalar(S, in + (size_t)b * S->n, out + (size_t)b * out_dist) != KO_OK);
+    (void)threads;
+    return bad ? KO_BUFFER_FAIL : KO_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* 2D                                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* kfft_math_transpose_cpx/_scalar (kfft_math.c:60-78):
+ * out[i*y + j] = in[j*x + i] for i < x, j < y. */
+static void
+ko_transpose_cpx(const ko_cpx* in, ko_cpx* out, uint32_t x, uint32_t y) {
+    for (size_t n = 0; n < (size_t)x * y; n++)
+        out[n] = in[(size_t)x * (n % y) + n / y];
+}
+static void
+ko_transpose_real(const double* in, double* out, uint32_t x, uint32_t y) {
+    for (size_t n = 0; n < (size_t)x * y; n++)
+        out[n] = in[(size_t)x * (n % y) + n / y];
+}
+
+/* kfft_eval2_cpx / kfft_2transform_normal (ext/2d/kfft_cpx2.c:92-124,
+ * :156-186): rows of length x, transpose, columns of length y, transpose. */
+int
+ko_eval2_cpx(uint32_t x, uint32_t y, uint32_t flags, const ko_cpx* in, ko_cpx* out) {
+    const uint32_t f = flags & ~(uint32_t)KO_FLAG_RENEW;
+    ko_plan* px = ko_plan_level(x, f, 0);
+    ko_plan* py = (x == y) ? px : ko_plan_level(y, f, 0);
+    const size_t n = (size_t)x * y;
+    ko_cpx* tmp = (ko_cpx*)calloc(n, sizeof(ko_cpx));
+    ko_cpx* dst = (ko_cpx*)calloc(n, sizeof(ko_cpx));
+    int ret = (px && py && tmp && dst) ? KO_OK : KO_BUFFER_FAIL;
+    if (ret == KO_OK) {
+        for (uint32_t i = 0; i < y && ret == KO_OK; i++)
+            ret = ko_eval_level(px, in + (size_t)x * i, tmp + (size_t)x * i);
+        ko_transpose_cpx(tmp, dst, x, y);
+        for (uint32_t i = 0; i < x && ret == KO_OK; i++)
+            ret = ko_eval_level(py, dst + (size_t)y * i, tmp + (size_t)y * i);
+        ko_transpose_cpx(tmp, dst, y, x);
+        memcpy(out, dst, sizeof(ko_cpx) * n);
+    }
+    free(tmp);
+    free(dst);
+    if (py != px)
+        ko_plan_free(py);
+    ko_plan_free(px);
+    return ret;
+}
+
+/* kfft_eval2_scalar / kfft_2transform (ext/2d/kfft_scalar2.c:109-148): real
+ * rows of length x with EXPAND_SCALAR, transpose, complex columns of length y,
+ * transpose.  Plans: ext/2d/kfft_scalar2.c:43-55. */
+int
+ko_eval2_scalar(uint32_t x, uint32_t y, uint32_t flags, const double* in, ko_cpx* out) {
+    if (flags & KO_FLAG_INVERSE)
+        return KO_IMPROPER_PLAN;
+    const uint32_t f = flags & ~(uint32_t)KO_FLAG_RENEW;
+    ko_splan* sx = ko_plan_scalar(x, f | KO_FLAG_EXPAND_SCALAR);
+    ko_plan* py = ko_plan_level(y, f, 0);
+    const size_t n = (size_t)x * y;
+    ko_cpx* tmp = (ko_cpx*)calloc(n, sizeof(ko_cpx));
+    ko_cpx* dst = (ko_cpx*)calloc(n, sizeof(ko_cpx));
+    int ret = (sx && py && tmp && dst) ? KO_OK : KO_BUFFER_FAIL;
+    if (ret == KO_OK) {
+        for (uint32_t i = 0; i < y && ret == KO_OK; i++)
+            ret = ko_eval_scalar(sx, in + (size_t)x * i, tmp + (size_t)x * i);
+        ko_transpose_cpx(tmp, dst, x, y);
+        for (uint32_t i = 0; i < x && ret == KO_OK; i++)
+            ret = ko_eval_level(py, dst + (size_t)y * i, tmp + (size_t)y * i);
+        ko_transpose_cpx(tmp, dst, y, x);
+        memcpy(out, dst, sizeof(ko_cpx) * n);
+    }
+    free(tmp);
+    free(dst);
+    ko_splan_free(sx);
+    ko_plan_free(py);
+    return ret;
+}
+
+/* kfft_evali2_scalar / kfft_2transform_inverse_normal
+ * (ext/2d/kfft_scalar2.c:193-246): complex inverse rows of length x,
+ * transpose, real inverse columns of length y, real transpose.  Plans:
+ * ext/2d/kfft_scalar2.c:31-42. */
+int
+ko_evali2_scalar(uint32_t x, uint32_t y, uint32_t flags, const ko_cpx* in, double* out) {
+    if (!(flags & KO_FLAG_INVERSE))
+        return KO_IMPROPER_PLAN;
+    const uint32_t f = flags & ~(uint32_t)KO_FLAG_RENEW;
+    ko_plan* px = ko_plan_level(x, f, 0);
+    ko_splan* sy = ko_plan_scalar(y, f | KO_FLAG_EXPAND_SCALAR);
+    const size_t n = (size_t)x * y;
+    ko_cpx* tmp = (ko_cpx*)calloc(n, sizeof(ko_cpx));
+    ko_cpx* tps = (ko_cpx*)calloc(n, sizeof(ko_cpx));
+    int ret = (px && sy && tmp && tps) ? KO_OK : KO_BUFFER_FAIL;
+    if (ret == KO_OK) {
+        for (uint32_t i = 0; i < y && ret == KO_OK; i++)
+            ret = ko_eval_level(px, in + (size_t)x * i, tmp + (size_t)x * i);
+        ko_transpose_cpx(tmp, tps, x, y);
+        double* re = (double*)tmp;
+        for (uint32_t i = 0; i < x && ret == KO_OK; i++)
+            ret = ko_evali_scalar(sy, tps + (size_t)y * i, re + (size_t)y * i);
+        ko_transpose_real(re, out, y, x);
+    }
+    free(tmp);
+    free(tps);
+    ko_plan_free(px);
+    ko_splan_free(sy);
+    return ret;
+}
+
+/* ------------------------------------------------------------------------ */
+/* convolution                                                              */
+/* ------------------------------------------------------------------------ */
+
+static void
+ko_hadamard(ko_cpx* acc, const ko_cpx* other, size_t n) {
+    for (size_t i = 0; i < n; i++)
+        acc[i] = ko_mul(acc[i], other[i]);
+}
+
+/* kfft_eval_conv_cpx (ext/conv/kfft_cpx_conv.c:75-104): forward plans with
+ * INVERSE and RENEW stripped (:22), pointwise product, inverse plan. */
+int
+ko_conv_cpx(uint32_t n, uint32_t flags, const ko_cpx* a, const ko_cpx* b, ko_cpx* out) {
+    const uint32_t f = flags & ~(uint32_t)(KO_FLAG_RENEW | KO_FLAG_INVERSE);
+    ko_plan* pf = ko_plan_level(n, f, 0);
+    ko_plan* pi = ko_plan_level(n, f | KO_FLAG_INVERSE, 0);
+    ko_cpx* ba = (ko_cpx*)calloc(n, sizeof(ko_cpx));
+    ko_cpx* bb = (ko_cpx*)calloc(n, sizeof(ko_cpx));
+    int ret = (pf && pi && ba && bb) ? KO_OK : KO_BUFFER_FAIL;
+    if (ret == KO_OK)
+        ret = ko_eval_level(pf, a, ba);
+    if (ret == KO_OK)
+        ret = ko_eval_level(pf, b, bb);
+    if (ret == KO_OK) {
+        ko_hadamard(bb, ba, n);
+        ret = ko_eval_level(pi, bb, out);
+    }
+    free(ba);
+    free(bb);
+    ko_plan_free(pf);
+    ko_plan_free(pi);
+    return ret;
+}
+
+/* kfft_eval_conv_scalar (ext/conv/kfft_scalar_conv.c:74-100).  The pointwise
+ * product covers n entries; for even n only the first n/2+1 carry spectrum
+ * and only those are read by the inverse.  The buffers are zeroed here where
+ * the reference leaves the unread tail uninitialised. */
+int
+ko_conv_scalar(uint32_t n, uint32_t flags, const double* a, const double* b, double* out) {
+    const uint32_t f = flags & ~(uint32_t)(KO_FLAG_RENEW | KO_FLAG_INVERSE);
+    ko_splan* pf = ko_plan_scalar(n, f);
+    ko_splan* pi = ko_plan_scalar(n, f | KO_FLAG_INVERSE);
+    ko_cpx* ba = (ko_cpx*)calloc(n, sizeof(ko_cpx));
+    ko_cpx* bb = (ko_cpx*)calloc(n, sizeof(ko_cpx));
+    int ret = (pf && pi && ba && bb) ? KO_OK : KO_BUFFER_FAIL;
+    if (ret == KO_OK)
+        ret = ko_eval_scalar(pf, a, ba);
+    if (ret == KO_OK)
+        ret = ko_eval_scalar(pf, b, bb);
+    if (ret == KO_OK) {
+        ko_hadamard(bb, ba, n);
+        ret = ko_evali_scalar(pi, bb, out);
+    }
+    free(ba);
+    free(bb);
+    ko_splan_free(pf);
+    ko_splan_free(pi);
+    return ret;
+}
+
+/* kfft_eval2_conv_cpx (ext/conv2d/kfft_cpx_conv.c:81-110).  The reference
+ * configures both 2D plans as x-by-x (plan->y = x at :67), so it is defined
+ * only for square inputs; other shapes are rejected here. */
+int
+ko_conv2_cpx(uint32_t x, uint32_t y, uint32_t flags, const ko_cpx* a, const ko_cpx* b,
+             ko_cpx* out) {
+    if (x != y)
+        return KO_BADARGUMENTS;
+    const uint32_t f = flags & ~(uint32_t)(KO_FLAG_RENEW | KO_FLAG_INVERSE);
+    const size_t n = (size_t)x * y;
+    ko_cpx* ba = (ko_cpx*)calloc(n, sizeof(ko_cpx));
+    ko_cpx* bb = (ko_cpx*)calloc(n, sizeof(ko_cpx));
+    int ret = (ba && bb) ? KO_OK : KO_BUFFER_FAIL;
+    if (ret == KO_OK)
+        ret = ko_eval2_cpx(x, x, f, a, ba);
+    if (ret == KO_OK)
+        ret = ko_eval2_cpx(x, x, f, b, bb);
+    if (ret == KO_OK) {
+        ko_hadamard(bb, ba, n);
+        ret = ko_eval2_cpx(x, x, f | KO_FLAG_INVERSE, bb, out);
+    }
+    free(ba);
+    free(bb);
+    return ret;
+}
+
+/* kfft_eval2_conv_scalar (ext/conv2d/kfft_scalar_conv.c:81-107). */
+int
+ko_conv2_scalar(uint32_t x, uint32_t y, uint32_t flags, const double* a, const double* b,
+                double* out) {
+    const uint32_t f = flags & ~(uint32_t)(KO_FLAG_RENEW | KO_FLAG_INVERSE);
+    const size_t n = (size_t)x * y;
+    ko_cpx* ba = (ko_cpx*)calloc(n, sizeof(ko_cpx));
+    ko_cpx* bb = (ko_cpx*)calloc(n, sizeof(ko_cpx));
+    int ret = (ba && bb) ? KO_OK : KO_BUFFER_FAIL;
+    if (ret == KO_OK)
+        ret = ko_eval2_scalar(x, y, f, a, ba);
+    if (ret == KO_OK)
+        ret = ko_eval2_scalar(x, y, f, b, bb);
+    if (ret == KO_OK) {
+        ko_hadamard(bb, ba, n);
+        ret = ko_evali2_scalar(x, y, f | KO_FLAG_INVERSE, bb, out);
+    }
+    free(ba);
+    free(bb);
+    return ret;
+}
+
+/* ------------------------------------------------------------------------ */
+/* independent check: direct DFT in long double with exact index reduction  */
+/* ------------------------------------------------------------------------ */
+void
+ko_dft_ld(const ko_cpx* in, ko_cpx* out, uint32_t n, int inverse) {
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    for (uint32_t k = 0; k < n; k++) {
+        long double ar = 0, ai = 0;
+        for (uint32_t j = 0; j < n; j++) {
+            uint64_t m = ((uint64_t)j * k) % n;
+            long double ph = (inverse ? two_pi : -two_pi) * (long double)m / n;
+            long double c = cosl(ph), s = sinl(ph);
+            ar += in[j].r * c - in[j].i * s;
+            ai += in[j].r * s + in[j].i * c;
+        }
+        out[k].r = (double)ar;
+        out[k].i = (double)ai;
+    }
+}

@@ -1,0 +1,90 @@
+/*
+ * CPU oracle for the kfft transform engine -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of libkfft 0.8.1's double-precision algorithm (the
+ * reference's default build: Rader on, RADER_LIMIT 11, PLAN_LEVEL 7,
+ * BFLY_LEVEL 5, libm-free binary-angle sincos, bisection sqrt).  It exists to
+ * check the GPU library and to time a CPU baseline.  Only tests/, the smoke
+ * entry point and the CPU-baseline leg of bench.py may load it; the product
+ * library never links or calls it.
+ *
+ * Every function cites the reference file:line it restates.  Symbols carry a
+ * `ko_` prefix so the oracle can share a process with the drop-in library,
+ * which exports the reference's own `kfft_*` names.
+ */
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    double r, i;
+} ko_cpx;
+
+/* Flag bits: identical values to incs/kfft_types.h:46-57. */
+enum {
+    KO_FLAG_INVERSE = 1 << 0,
+    KO_FLAG_RENEW = 1 << 1,
+    KO_FLAG_GENERIC = 1 << 2,
+    KO_FLAG_GENERIC_ONLY = 1 << 3,
+    KO_FLAG_EXPAND_SCALAR = 1 << 4,
+    KO_FLAG_DISABLE_NORM = 1 << 5,
+};
+
+/* Build constants of the reference default build (meson_options.txt:42-47). */
+#define KO_RADER_LIMIT 11
+#define KO_PLAN_LEVEL 7
+#define KO_BFLY_LEVEL 5
+
+/* ---- math primitives ---------------------------------------------------- */
+double ko_sqrt_bisect(double x);                        /* math/kfft_custom_math.h:43-67 */
+void ko_sincos(double x, double* co, double* si);       /* math/kfft_scd.c:13-103 */
+ko_cpx ko_twiddle(uint32_t i, uint32_t n, int inverse); /* incs/kfft_math.h:206-216 */
+uint32_t ko_modpow(uint32_t x, uint32_t y, uint32_t m); /* incs/kfft_math.h:172-180 */
+uint32_t ko_prim_root(uint32_t p);                      /* kfft_math.c:5-33 */
+uint32_t ko_prim_root_inv(uint32_t p, uint32_t g);      /* kfft_math.c:35-38 */
+uint32_t ko_next_fast_size(uint32_t n);                 /* kfft_ext.c:38-52 */
+
+/* kf_factor (kfft_cpx.c:103-143): fills fac[2*k]=p_k, fac[2*k+1]=m_k and the
+ * list of Rader-registered primes; returns the factor count. */
+int ko_factor(uint32_t n, uint32_t level, uint32_t* fac, uint32_t* primes, uint32_t* nprimes);
+
+/* ---- 1D complex (kfft_cpx.c, kfft_work.c, kfft_bfly.c, kfft_generic.c, kfft_rader.c) */
+typedef struct ko_plan ko_plan;
+ko_plan* ko_plan_cpx(uint32_t n, uint32_t flags);
+void ko_plan_free(ko_plan* p);
+int ko_eval_cpx(const ko_plan* p, const ko_cpx* in, ko_cpx* out);
+/* batch of contiguous transforms, `threads` OpenMP threads (<=0: runtime default) */
+int ko_eval_cpx_batch(const ko_plan* p, const ko_cpx* in, ko_cpx* out, uint32_t batch, int threads);
+
+/* ---- 1D real / "scalar" (kfft_scalar.c) -------------------------------- */
+typedef struct ko_splan ko_splan;
+ko_splan* ko_plan_scalar(uint32_t n, uint32_t flags);
+void ko_splan_free(ko_splan* p);
+int ko_eval_scalar(const ko_splan* p, const double* in, ko_cpx* out);
+int ko_evali_scalar(const ko_splan* p, const ko_cpx* in, double* out);
+int ko_eval_scalar_batch(const ko_splan* p, const double* in, ko_cpx* out, uint32_t batch,
+                         size_t out_dist, int threads);
+
+/* ---- 2D (ext/2d) --------------------------------------------------------- */
+int ko_eval2_cpx(uint32_t x, uint32_t y, uint32_t flags, const ko_cpx* in, ko_cpx* out);
+int ko_eval2_scalar(uint32_t x, uint32_t y, uint32_t flags, const double* in, ko_cpx* out);
+int ko_evali2_scalar(uint32_t x, uint32_t y, uint32_t flags, const ko_cpx* in, double* out);
+
+/* ---- convolution (ext/conv, ext/conv2d) --------------------------------- */
+int ko_conv_cpx(uint32_t n, uint32_t flags, const ko_cpx* a, const ko_cpx* b, ko_cpx* out);
+int ko_conv_scalar(uint32_t n, uint32_t flags, const double* a, const double* b, double* out);
+int ko_conv2_cpx(uint32_t x, uint32_t y, uint32_t flags, const ko_cpx* a, const ko_cpx* b,
+                 ko_cpx* out);
+int ko_conv2_scalar(uint32_t x, uint32_t y, uint32_t flags, const double* a, const double* b,
+                    double* out);
+
+/* ---- plain O(N^2) DFT in long double, an independent check ------------- */
+void ko_dft_ld(const ko_cpx* in, ko_cpx* out, uint32_t n, int inverse);
+
+#ifdef __cplusplus
+}
+#endif

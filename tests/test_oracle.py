@@ -1,0 +1,259 @@
+"""Pins for the CPU oracle (oracle/kfft_oracle.c) before it is trusted.
+
+Each check is independent of the oracle's own formulas: the reference's
+known-answer vectors, a long-double direct DFT, numpy's pocketfft, closed
+forms, and mathematical invariants.  Shapes are chosen non-square and lengths
+odd/even/prime so that a transposed index, a dropped term or a wrong sign
+fails at least one of them.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def cplx(rng, *shape):
+    return rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)
+
+
+# ---------------------------------------------------------------- golden vectors
+@pytest.fixture(scope="module")
+def defs_cpx1():
+    with open(os.path.join(HERE, "golden", "defs_cpx1.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 7, 330])
+def test_golden_forward_inverse(defs_cpx1, n):
+    """test/unit/suites/api_cpx1.c:157-225 at its tolerance 1e-3 (values printed to 4 decimals)."""
+    g = defs_cpx1[str(n)]
+    xin = np.array(g["in"]) @ np.array([1, 1j])
+    xout = np.array(g["out"]) @ np.array([1, 1j])
+    assert np.abs(O.fft(xin) - xout).max() < 1e-3
+    assert np.abs(O.fft(xout, O.FLAG_INVERSE) - xin).max() < 1e-3
+
+
+# ---------------------------------------------------------------- 1D complex
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 9, 11, 12, 13, 16, 22, 25, 30, 49, 60, 64,
+                               97, 100, 128, 210, 330, 360, 512, 1000, 1008, 1009])
+def test_cpx_vs_longdouble_dft(n):
+    rng = np.random.default_rng(100 + n)
+    x = cplx(rng, n)
+    ref = O.dft_ld(x)
+    assert rel(O.fft(x), ref) < 1e-13
+    # inverse is the conjugate transform divided by n (kfft_cpx.c:351-363)
+    assert rel(O.fft(x, O.FLAG_INVERSE), O.dft_ld(x, inverse=True) / n) < 1e-13
+
+
+@pytest.mark.parametrize("n", [4096, 6561, 4160, 2080, 65537, 1 << 16, 94132])
+def test_cpx_vs_numpy_large(n):
+    rng = np.random.default_rng(n)
+    x = cplx(rng, n)
+    assert rel(O.fft(x), np.fft.fft(x)) < 1e-12 * math.log2(n)
+
+
+@pytest.mark.parametrize("n", [7, 64, 330, 1009])
+def test_ramp_closed_form(n):
+    """X[0] = n(n+1)/2, X[k] = -n/2 + i (n/2) cot(pi k / n) for x = 1..n."""
+    x = np.arange(1, n + 1, dtype=np.complex128)
+    k = np.arange(1, n)
+    ref = np.empty(n, np.complex128)
+    ref[0] = n * (n + 1) / 2
+    ref[1:] = -n / 2 + 1j * (n / 2) / np.tan(np.pi * k / n)
+    assert rel(O.fft(x), ref) < 1e-12
+
+
+@pytest.mark.parametrize("n", [60, 1009, 4096])
+def test_normalisation_and_roundtrip(n):
+    rng = np.random.default_rng(7)
+    x = cplx(rng, n)
+    X = O.fft(x)
+    assert rel(O.fft(X, O.FLAG_INVERSE), x) < 1e-12
+    assert rel(O.fft(X, O.FLAG_INVERSE | O.FLAG_DISABLE_NORM), n * x) < 1e-12
+    # Parseval and linearity
+    assert abs(np.vdot(X, X).real / n - np.vdot(x, x).real) / np.vdot(x, x).real < 1e-12
+    y = cplx(rng, n)
+    assert rel(O.fft(2 * x - 3j * y), 2 * X - 3j * O.fft(y)) < 1e-12
+
+
+@pytest.mark.parametrize("flags", [O.FLAG_GENERIC, O.FLAG_GENERIC_ONLY])
+@pytest.mark.parametrize("n", [13, 97, 330])
+def test_generic_flags(flags, n):
+    """GENERIC skips Rader, GENERIC_ONLY is one O(n^2) DFT (kfft_generic.c:51-55, kfft_work.c:7-8)."""
+    rng = np.random.default_rng(n)
+    x = cplx(rng, n)
+    assert rel(O.fft(x, flags), O.dft_ld(x)) < 1e-13
+
+
+def test_in_place_batch_matches_single():
+    rng = np.random.default_rng(3)
+    x = cplx(rng, 5, 330)
+    p = O.Plan(330)
+    out = p(x)
+    for b in range(5):
+        assert np.array_equal(out[b], p(x[b]))
+
+
+# ---------------------------------------------------------------- planner pins
+@pytest.mark.parametrize("n,expect,primes", [
+    (4096, [4] * 6, []),
+    (1000, [4, 2, 5, 5, 5], []),
+    (6561, [3] * 8, []),
+    (1009, [1009], [1009]),
+    (1008, [4, 4, 3, 3, 7], []),
+    (65537, [65537], [65537]),
+    (1 << 20, [4] * 10, []),
+    (1 << 17, [4] * 8 + [2], []),
+    (8192, [4] * 6 + [2], []),
+    (4160, [4, 4, 4, 5, 13], [13]),
+    (2080, [4, 4, 2, 5, 13], [13]),
+    (330, [2, 3, 5, 11], [11]),
+    (121, [121], [121]),  # bisection sqrt(121) truncates to 10: the reference keeps 121 whole
+])
+def test_factor_lists(n, expect, primes):
+    """kf_factor (kfft_cpx.c:103-143) factor lists, checked by an independent restatement."""
+    pairs, pr = O.factor(n)
+    assert [p for p, _ in pairs] == expect
+    m = n
+    for p, rest in pairs:
+        m //= p
+        assert rest == m
+    assert pr == primes
+    assert int(O.sqrt_bisect(n)) in (math.isqrt(n), math.isqrt(n) - 1)
+
+
+@pytest.mark.parametrize("n", [121, 169, 529, 841])
+def test_reference_defect_prime_squares(n):
+    """N = p^2 whose bisection sqrt truncates below p is run as one 'prime' through
+    Rader on a composite modulus; the reference's output there is not a DFT.  The
+    oracle restates that faithfully; the GPU library does not inherit it."""
+    pairs, pr = O.factor(n)
+    assert pairs == [(n, 1)] and pr == [n]
+    rng = np.random.default_rng(n)
+    x = cplx(rng, n)
+    assert rel(O.fft(x), np.fft.fft(x)) > 0.1
+    assert rel(O.fft(x, O.FLAG_GENERIC), np.fft.fft(x)) < 1e-12
+
+
+def test_twiddles_and_number_theory():
+    for n in [3, 7, 1000, 4096, 65537]:
+        for i in [0, 1, n // 3, n // 2, n - 1]:
+            w = O.twiddle(i, n)
+            assert abs(w - np.exp(-2j * np.pi * i / n)) < 5e-14
+            assert abs(O.twiddle(i, n, True) - np.conj(w)) < 5e-14
+    c, s = O.sincos(-2.5)
+    assert abs(c - math.cos(-2.5)) < 5e-14 and abs(s - math.sin(-2.5)) < 5e-14
+    for p, g in [(11, 2), (13, 2), (1009, 11), (65537, 3), (7, 3), (233, 3)]:
+        assert O.prim_root(p) == g
+        assert (O.prim_root_inv(p, g) * g) % p == 1
+        assert len({O.modpow(g, k, p) for k in range(p - 1)}) == p - 1
+    def smooth(m):
+        for q in (2, 3, 5):
+            while m % q == 0:
+                m //= q
+        return m == 1
+
+    for n in [1, 7, 11, 97, 1009, 4097, 65537]:
+        assert O.next_fast_size(n) == next(m for m in range(n, 4 * n) if smooth(m))
+
+
+# ---------------------------------------------------------------- real input
+@pytest.mark.parametrize("n", [2, 3, 8, 9, 30, 31, 100, 1000, 2080, 4160])
+def test_scalar_forward(n):
+    rng = np.random.default_rng(n)
+    x = rng.uniform(-1, 1, n)
+    X = O.rfft(x)
+    if n % 2 == 0:
+        assert X.shape == (n // 2 + 1,)
+        assert rel(X, np.fft.rfft(x)) < 1e-13
+        assert X[0].imag == 0 and X[n // 2].imag == 0
+    else:
+        assert X.shape == (n,)
+        assert rel(X, np.fft.fft(x)) < 1e-13
+    assert rel(O.rfft(x, O.FLAG_EXPAND_SCALAR), np.fft.fft(x)) < 1e-13
+
+
+@pytest.mark.parametrize("n", [2, 3, 8, 9, 30, 31, 1000, 2080])
+def test_scalar_inverse_scaling(n):
+    """Normalised inverse gives 2x for even n (rebuild_inverse builds 2Z,
+    kfft_scalar.c:224-240) and x for odd n; DISABLE_NORM gives n*x."""
+    rng = np.random.default_rng(n + 1)
+    x = rng.uniform(-1, 1, n)
+    X = np.fft.rfft(x) if n % 2 == 0 else np.fft.fft(x)
+    k = 2.0 if n % 2 == 0 else 1.0
+    assert rel(O.irfft(X, n), k * x) < 1e-13
+    assert rel(O.irfft(X, n, O.FLAG_INVERSE | O.FLAG_DISABLE_NORM), n * x) < 1e-13
+
+
+def test_scalar_n2_closed_form():
+    assert np.allclose(O.irfft(O.rfft([0.25, -3.0]), 2), [0.5, -6.0], rtol=0, atol=0)
+
+
+# ---------------------------------------------------------------- 2D
+@pytest.mark.parametrize("y,x", [(6, 10), (7, 5), (16, 12), (11, 13)])
+def test_2d_complex(y, x):
+    rng = np.random.default_rng(x * y)
+    a = cplx(rng, y, x)
+    A = O.fft2(a)
+    assert rel(A, np.fft.fft2(a)) < 1e-13
+    assert rel(O.fft2(A, O.FLAG_INVERSE), a) < 1e-13
+
+
+@pytest.mark.parametrize("y,x", [(6, 10), (7, 10), (6, 9), (9, 7)])
+def test_2d_scalar(y, x):
+    rng = np.random.default_rng(x + 100 * y)
+    a = rng.uniform(-1, 1, (y, x))
+    A = O.rfft2(a)
+    assert rel(A, np.fft.fft2(a)) < 1e-13
+    k = 2.0 if y % 2 == 0 else 1.0
+    assert rel(O.irfft2(A), k * a) < 1e-13
+    assert rel(O.irfft2(A, O.FLAG_INVERSE | O.FLAG_DISABLE_NORM), x * y * a) < 1e-13
+
+
+# ---------------------------------------------------------------- convolution
+def _circ(a, b):
+    n = len(a)
+    return np.array([sum(a[j] * b[(k - j) % n] for j in range(n)) for k in range(n)])
+
+
+@pytest.mark.parametrize("n", [12, 13, 30])
+def test_conv(n):
+    rng = np.random.default_rng(n)
+    a, b = cplx(rng, n), cplx(rng, n)
+    assert rel(O.conv(a, b), _circ(a, b)) < 1e-13
+    ar, br = a.real.copy(), b.real.copy()
+    k = 2.0 if n % 2 == 0 else 1.0
+    assert rel(O.conv_scalar(ar, br), k * _circ(ar, br)) < 1e-13
+
+
+def _circ2(a, b):
+    y, x = a.shape
+    out = np.zeros_like(a)
+    for j in range(y):
+        for i in range(x):
+            out += a[j, i] * np.roll(np.roll(b, j, axis=0), i, axis=1)
+    return out
+
+
+@pytest.mark.parametrize("y,x", [(8, 8), (6, 10), (7, 6)])
+def test_conv2(y, x):
+    rng = np.random.default_rng(x * y + 5)
+    a, b = rng.uniform(-1, 1, (y, x)), rng.uniform(-1, 1, (y, x))
+    k = 2.0 if y % 2 == 0 else 1.0
+    assert rel(O.conv2_scalar(a, b), k * _circ2(a, b)) < 1e-13
+    if x == y:
+        ac, bc = cplx(rng, y, x), cplx(rng, y, x)
+        assert rel(O.conv2(ac, bc), _circ2(ac, bc)) < 1e-13
+    else:
+        with pytest.raises(RuntimeError):
+            O.conv2(a + 0j, b + 0j)  # undefined in the reference (ext/conv2d/kfft_cpx_conv.c:67)
