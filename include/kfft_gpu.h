@@ -1,0 +1,45 @@
+/*
+ * GPU additions of the MI355X drop-in.  The reference has no counterpart: it
+ * has neither a batch API nor device memory.  Everything here is additive;
+ * the reference's entry points keep their signatures and semantics.
+ *
+ * Pointers: every eval entry point of kfft.h accepts host or device memory.
+ * Device (or managed) buffers are transformed in place on the GPU; host
+ * buffers are staged through device memory (correct but PCIe-bound).
+ * The *_async variants take device pointers only and return without
+ * synchronising; `stream` is a hipStream_t (NULL: the library's stream).
+ */
+#pragma once
+
+/* device memory and synchronisation */
+KFFT_API void* kfft_gpu_malloc(size_t bytes);
+KFFT_API void kfft_gpu_free(void* ptr);
+KFFT_API kfft_return_t kfft_gpu_memcpy_h2d(void* dst, const void* src, size_t bytes);
+KFFT_API kfft_return_t kfft_gpu_memcpy_d2h(void* dst, const void* src, size_t bytes);
+KFFT_API kfft_return_t kfft_gpu_synchronize(void);
+KFFT_API void* kfft_gpu_stream(void);
+KFFT_API const char* kfft_gpu_last_error(void);
+/* one line describing how a complex, scalar or 2D plan is executed */
+KFFT_API const char* kfft_gpu_plan_describe(const void* plan);
+
+/* `batch` contiguous transforms: complex in/out distance nfft */
+KFFT_API kfft_return_t kfft_eval_cpx_batch(kfft_plan_cpx* plan, const kfft_cpx* fin,
+                                           kfft_cpx* fout, uint32_t batch);
+KFFT_API kfft_return_t kfft_eval_cpx_batch_async(kfft_plan_cpx* plan, const kfft_cpx* fin,
+                                                 kfft_cpx* fout, uint32_t batch, void* stream);
+
+/* real input: input distance nfft reals; output distance kfft_scalar_out_len(plan) */
+KFFT_API uint32_t kfft_scalar_out_len(const kfft_plan_sclr* plan);
+KFFT_API kfft_return_t kfft_eval_scalar_batch(kfft_plan_sclr* plan, const kfft_scalar* fin,
+                                              kfft_cpx* fout, uint32_t batch);
+KFFT_API kfft_return_t kfft_evali_scalar_batch(kfft_plan_sclr* plan, const kfft_cpx* fin,
+                                               kfft_scalar* fout, uint32_t batch);
+KFFT_API kfft_return_t kfft_eval_scalar_batch_async(kfft_plan_sclr* plan, const kfft_scalar* fin,
+                                                    kfft_cpx* fout, uint32_t batch, void* stream);
+KFFT_API kfft_return_t kfft_evali_scalar_batch_async(kfft_plan_sclr* plan, const kfft_cpx* fin,
+                                                     kfft_scalar* fout, uint32_t batch,
+                                                     void* stream);
+
+/* 2D, asynchronous on device pointers */
+KFFT_API kfft_return_t kfft_eval2_cpx_async(kfft_plan_c2d* plan, const kfft_cpx* fin,
+                                            kfft_cpx* fout, void* stream);

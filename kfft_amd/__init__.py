@@ -1,0 +1,367 @@
+"""Python mirror of the kfft.h C ABI of the MI355X transform engine.
+
+The product is the C-ABI shared library (``kfft_amd/lib/libkfft_amd.so`` for
+``kfft_scalar = double``, ``libkfft_amd_half.so`` for ``float``).  This module
+binds it with ctypes so that tests and the benchmark can drive exactly the
+entry points a C caller uses.  Buffers may be numpy arrays (host memory, staged
+by the library) or torch tensors on the GPU (device memory, used in place).
+
+There is no CPU fallback: if the library is missing or no GPU is visible the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "lib")
+
+FLAG_NORMAL = 0
+FLAG_INVERSE = 1 << 0
+FLAG_RENEW = 1 << 1
+FLAG_GENERIC = 1 << 2
+FLAG_GENERIC_ONLY = 1 << 3
+FLAG_EXPAND_SCALAR = 1 << 4
+FLAG_DISABLE_NORM = 1 << 5
+FLAG_GPU_SPLIT_F16 = 1 << 8
+FLAG_GPU_SPLIT_BF16 = 1 << 9
+FLAG_GPU_REFINE = 1 << 10
+
+RET_SUCCESS, RET_ALLOC_FAIL, RET_BUFFER_FAIL, RET_FREE_NULL = 0, 1, 2, 3
+RET_IMPROPER_PLAN, RET_BADARGUMENTS, RET_GPU_FAIL = 4, 5, 6
+
+# every symbol the public headers declare (include/*.h)
+EXPORTS = [
+    "kfft_pool_create", "kfft_pool_alloc", "kfft_pool_empty", "kfft_pool_zmem",
+    "kfft_pool_clear", "kfft_pool_free",
+    "kfft_info", "kfft_next_fast_size", "kfft_malloc", "kfft_free_null", "kfft_cleanup",
+    "kfft_strerr",
+    "kfft_config_cpx", "kfft_eval_cpx",
+    "kfft_config_scalar", "kfft_eval_scalar", "kfft_evali_scalar",
+    "kfft_eval_scalar_internal", "kfft_evali_scalar_internal",
+    "kfft_math_hadamard_cpx", "kfft_math_hadamard_scalar", "kfft_math_transpose_cpx",
+    "kfft_math_transpose_scalar",
+    "kfft_shift_cpx", "kfft_shift_scalar",
+    "kfft_config2_cpx", "kfft_eval2_cpx", "kfft_shift2_cpx",
+    "kfft_config2_scalar", "kfft_eval2_scalar", "kfft_evali2_scalar", "kfft_shift2_scalar",
+    "kfft_config_conv_cpx", "kfft_eval_conv_cpx", "kfft_config_conv_scalar",
+    "kfft_eval_conv_scalar", "kfft_config2_conv_cpx", "kfft_eval2_conv_cpx",
+    "kfft_config2_conv_scalar", "kfft_eval2_conv_scalar",
+    "kfft_gpu_malloc", "kfft_gpu_free", "kfft_gpu_memcpy_h2d", "kfft_gpu_memcpy_d2h",
+    "kfft_gpu_synchronize", "kfft_gpu_stream", "kfft_gpu_last_error", "kfft_gpu_plan_describe",
+    "kfft_eval_cpx_batch", "kfft_eval_cpx_batch_async",
+    "kfft_scalar_out_len", "kfft_eval_scalar_batch", "kfft_evali_scalar_batch",
+    "kfft_eval_scalar_batch_async", "kfft_evali_scalar_batch_async",
+    "kfft_eval2_cpx_async",
+]
+
+
+class KfftError(RuntimeError):
+    pass
+
+
+class _Info(C.Structure):
+    _fields_ = [("vmajor", C.c_uint16), ("vminor", C.c_uint16), ("vpatch", C.c_uint16),
+                ("flags", C.c_uint16)]
+
+
+def lib_path(half: bool = False) -> str:
+    return os.path.join(LIB_DIR, "libkfft_amd_half.so" if half else "libkfft_amd.so")
+
+
+_libs: dict = {}
+
+
+def load(half: bool = False):
+    """Load the C-ABI library (double or float build) with local symbols."""
+    if half in _libs:
+        return _libs[half]
+    path = lib_path(half)
+    if not os.path.exists(path):
+        raise KfftError(f"{path} is not built; run __graft_entry__.build() or python kfft_amd/build.py")
+    L = C.CDLL(path, mode=C.RTLD_LOCAL)
+    vp, u32, sz = C.c_void_p, C.c_uint32, C.c_size_t
+    ret = C.c_uint
+    sigs = {
+        "kfft_pool_create": (vp, [sz]),
+        "kfft_pool_alloc": (vp, [vp, sz]),
+        "kfft_pool_empty": (sz, [vp]),
+        "kfft_pool_clear": (None, [vp]),
+        "kfft_pool_free": (None, [vp]),
+        "kfft_info": (None, [vp]),
+        "kfft_next_fast_size": (u32, [u32]),
+        "kfft_malloc": (vp, [u32]),
+        "kfft_free_null": (None, [vp]),
+        "kfft_cleanup": (ret, [vp]),
+        "kfft_strerr": (C.c_char_p, [ret]),
+        "kfft_config_cpx": (vp, [u32, u32, vp, vp]),
+        "kfft_eval_cpx": (ret, [vp, vp, vp]),
+        "kfft_config_scalar": (vp, [u32, u32, vp, vp]),
+        "kfft_eval_scalar": (ret, [vp, vp, vp]),
+        "kfft_evali_scalar": (ret, [vp, vp, vp]),
+        "kfft_config2_cpx": (vp, [u32, u32, u32, vp, vp]),
+        "kfft_eval2_cpx": (ret, [vp, vp, vp]),
+        "kfft_config2_scalar": (vp, [u32, u32, u32, vp, vp]),
+        "kfft_eval2_scalar": (ret, [vp, vp, vp]),
+        "kfft_evali2_scalar": (ret, [vp, vp, vp]),
+        "kfft_config_conv_cpx": (vp, [u32, u32, vp, vp]),
+        "kfft_eval_conv_cpx": (ret, [vp, vp, vp, vp]),
+        "kfft_config_conv_scalar": (vp, [u32, u32, vp, vp]),
+        "kfft_eval_conv_scalar": (ret, [vp, vp, vp, vp]),
+        "kfft_config2_conv_cpx": (vp, [u32, u32, u32, vp, vp]),
+        "kfft_eval2_conv_cpx": (ret, [vp, vp, vp, vp]),
+        "kfft_config2_conv_scalar": (vp, [u32, u32, u32, vp, vp]),
+        "kfft_eval2_conv_scalar": (ret, [vp, vp, vp, vp]),
+        "kfft_math_hadamard_cpx": (None, [vp, vp, u32]),
+        "kfft_math_transpose_cpx": (None, [vp, vp, u32, u32]),
+        "kfft_shift_cpx": (None, [vp, u32, C.c_bool, vp]),
+        "kfft_shift_scalar": (None, [vp, u32, C.c_bool, vp]),
+        "kfft_shift2_cpx": (None, [vp, vp, u32, u32, C.c_bool, vp]),
+        "kfft_gpu_malloc": (vp, [sz]),
+        "kfft_gpu_free": (None, [vp]),
+        "kfft_gpu_synchronize": (ret, []),
+        "kfft_gpu_stream": (vp, []),
+        "kfft_gpu_last_error": (C.c_char_p, []),
+        "kfft_gpu_plan_describe": (C.c_char_p, [vp]),
+        "kfft_eval_cpx_batch": (ret, [vp, vp, vp, u32]),
+        "kfft_eval_cpx_batch_async": (ret, [vp, vp, vp, u32, vp]),
+        "kfft_scalar_out_len": (u32, [vp]),
+        "kfft_eval_scalar_batch": (ret, [vp, vp, vp, u32]),
+        "kfft_evali_scalar_batch": (ret, [vp, vp, vp, u32]),
+        "kfft_eval_scalar_batch_async": (ret, [vp, vp, vp, u32, vp]),
+        "kfft_evali_scalar_batch_async": (ret, [vp, vp, vp, u32, vp]),
+        "kfft_eval2_cpx_async": (ret, [vp, vp, vp, vp]),
+    }
+    for name, (res, args) in sigs.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _libs[half] = L
+    return L
+
+
+def info(half: bool = False):
+    i = _Info()
+    load(half).kfft_info(C.byref(i))
+    return (i.vmajor, i.vminor, i.vpatch), i.flags
+
+
+def last_error(half: bool = False) -> str:
+    return (load(half).kfft_gpu_last_error() or b"").decode()
+
+
+def check(code: int, what: str, half: bool = False) -> None:
+    if code != RET_SUCCESS:
+        msg = load(half).kfft_strerr(code).decode()
+        raise KfftError(f"{what}: {msg} ({code}) {last_error(half)}")
+
+
+# ----------------------------------------------------------------- buffers
+def _ptr(a):
+    """Raw pointer of a numpy array or torch tensor (must be contiguous)."""
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"]
+        return C.c_void_p(a.ctypes.data)
+    assert a.is_contiguous()
+    return C.c_void_p(a.data_ptr())
+
+
+def _is_torch(a) -> bool:
+    return type(a).__module__.startswith("torch")
+
+
+def _empty_like_kind(ref, shape, complex_out: bool, half: bool):
+    if _is_torch(ref):
+        import torch
+
+        dt = (torch.complex64 if half else torch.complex128) if complex_out else (
+            torch.float32 if half else torch.float64)
+        return torch.empty(shape, dtype=dt, device=ref.device)
+    dt = (np.complex64 if half else np.complex128) if complex_out else (
+        np.float32 if half else np.float64)
+    return np.empty(shape, dtype=dt)
+
+
+def _as_input(a, complex_in: bool, half: bool):
+    if _is_torch(a):
+        return a.contiguous()
+    dt = (np.complex64 if half else np.complex128) if complex_in else (
+        np.float32 if half else np.float64)
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+# ------------------------------------------------------------------- plans
+class _Plan:
+    def __init__(self, half: bool):
+        self.half = half
+        self.L = load(half)
+        self._p = None
+
+    def __del__(self):
+        try:
+            if self._p:
+                self.L.kfft_cleanup(self._p)
+                self._p = None
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._p
+
+
+class CpxPlan(_Plan):
+    """kfft_config_cpx / kfft_eval_cpx (+ batch)."""
+
+    def __init__(self, n: int, flags: int = 0, half: bool = False):
+        super().__init__(half)
+        self.n, self.flags = n, flags
+        self._p = self.L.kfft_config_cpx(n, flags, None, None)
+        if not self._p:
+            raise KfftError(f"kfft_config_cpx({n}) failed: {last_error(half)}")
+
+    def describe(self) -> str:
+        return self.L.kfft_gpu_plan_describe(self._p).decode()
+
+    def __call__(self, x, out=None):
+        x = _as_input(x, True, self.half)
+        assert x.shape[-1] == self.n
+        batch = int(np.prod(x.shape[:-1])) if len(x.shape) > 1 else 1
+        if out is None:
+            out = _empty_like_kind(x, tuple(x.shape), True, self.half)
+        if batch == 1:
+            check(self.L.kfft_eval_cpx(self._p, _ptr(x), _ptr(out)), "kfft_eval_cpx", self.half)
+        else:
+            check(self.L.kfft_eval_cpx_batch(self._p, _ptr(x), _ptr(out), batch),
+                  "kfft_eval_cpx_batch", self.half)
+        return out
+
+    def run_async(self, x, out, batch: int, stream=None):
+        check(self.L.kfft_eval_cpx_batch_async(self._p, _ptr(x), _ptr(out), batch, stream),
+              "kfft_eval_cpx_batch_async", self.half)
+
+
+class ScalarPlan(_Plan):
+    """kfft_config_scalar / kfft_eval_scalar / kfft_evali_scalar (+ batch)."""
+
+    def __init__(self, n: int, flags: int = 0, half: bool = False):
+        super().__init__(half)
+        self.n, self.flags = n, flags
+        self._p = self.L.kfft_config_scalar(n, flags, None, None)
+        if not self._p:
+            raise KfftError(f"kfft_config_scalar({n}) failed: {last_error(half)}")
+        self.out_len = self.L.kfft_scalar_out_len(self._p)
+
+    def forward(self, x, out=None):
+        x = _as_input(x, False, self.half)
+        batch = int(np.prod(x.shape[:-1])) if len(x.shape) > 1 else 1
+        if out is None:
+            out = _empty_like_kind(x, tuple(x.shape[:-1]) + (self.out_len,), True, self.half)
+        if batch == 1:
+            code = self.L.kfft_eval_scalar(self._p, _ptr(x), _ptr(out))
+        else:
+            code = self.L.kfft_eval_scalar_batch(self._p, _ptr(x), _ptr(out), batch)
+        check(code, "kfft_eval_scalar", self.half)
+        return out
+
+    def inverse(self, X, out=None):
+        X = _as_input(X, True, self.half)
+        assert X.shape[-1] == self.out_len
+        batch = int(np.prod(X.shape[:-1])) if len(X.shape) > 1 else 1
+        if out is None:
+            out = _empty_like_kind(X, tuple(X.shape[:-1]) + (self.n,), False, self.half)
+        if batch == 1:
+            code = self.L.kfft_evali_scalar(self._p, _ptr(X), _ptr(out))
+        else:
+            code = self.L.kfft_evali_scalar_batch(self._p, _ptr(X), _ptr(out), batch)
+        check(code, "kfft_evali_scalar", self.half)
+        return out
+
+
+class Cpx2Plan(_Plan):
+    def __init__(self, x: int, y: int, flags: int = 0, half: bool = False):
+        super().__init__(half)
+        self.x, self.y, self.flags = x, y, flags
+        self._p = self.L.kfft_config2_cpx(x, y, flags, None, None)
+        if not self._p:
+            raise KfftError(f"kfft_config2_cpx({x},{y}) failed: {last_error(half)}")
+
+    def __call__(self, a, out=None):
+        a = _as_input(a, True, self.half)
+        assert tuple(a.shape) == (self.y, self.x)
+        if out is None:
+            out = _empty_like_kind(a, (self.y, self.x), True, self.half)
+        check(self.L.kfft_eval2_cpx(self._p, _ptr(a), _ptr(out)), "kfft_eval2_cpx", self.half)
+        return out
+
+
+class Scalar2Plan(_Plan):
+    def __init__(self, x: int, y: int, flags: int = 0, half: bool = False):
+        super().__init__(half)
+        self.x, self.y, self.flags = x, y, flags
+        self._p = self.L.kfft_config2_scalar(x, y, flags, None, None)
+        if not self._p:
+            raise KfftError(f"kfft_config2_scalar({x},{y}) failed: {last_error(half)}")
+
+    def forward(self, a, out=None):
+        a = _as_input(a, False, self.half)
+        if out is None:
+            out = _empty_like_kind(a, (self.y, self.x), True, self.half)
+        check(self.L.kfft_eval2_scalar(self._p, _ptr(a), _ptr(out)), "kfft_eval2_scalar",
+              self.half)
+        return out
+
+    def inverse(self, A, out=None):
+        A = _as_input(A, True, self.half)
+        if out is None:
+            out = _empty_like_kind(A, (self.y, self.x), False, self.half)
+        check(self.L.kfft_evali2_scalar(self._p, _ptr(A), _ptr(out)), "kfft_evali2_scalar",
+              self.half)
+        return out
+
+
+class ConvPlan(_Plan):
+    """1D convolution plans: complex (kfft_config_conv_cpx) or real (kfft_config_conv_scalar)."""
+
+    def __init__(self, n: int, flags: int = 0, real: bool = False, half: bool = False):
+        super().__init__(half)
+        self.n, self.real = n, real
+        cfg = self.L.kfft_config_conv_scalar if real else self.L.kfft_config_conv_cpx
+        self._p = cfg(n, flags, None, None)
+        if not self._p:
+            raise KfftError(f"conv config({n}) failed: {last_error(half)}")
+
+    def __call__(self, a, b, out=None):
+        a = _as_input(a, not self.real, self.half)
+        b = _as_input(b, not self.real, self.half)
+        if out is None:
+            out = _empty_like_kind(a, tuple(a.shape), not self.real, self.half)
+        ev = self.L.kfft_eval_conv_scalar if self.real else self.L.kfft_eval_conv_cpx
+        check(ev(self._p, _ptr(a), _ptr(b), _ptr(out)), "kfft_eval_conv", self.half)
+        return out
+
+
+class Conv2Plan(_Plan):
+    def __init__(self, x: int, y: int, flags: int = 0, real: bool = False, half: bool = False):
+        super().__init__(half)
+        self.x, self.y, self.real = x, y, real
+        cfg = self.L.kfft_config2_conv_scalar if real else self.L.kfft_config2_conv_cpx
+        self._p = cfg(x, y, flags, None, None)
+        if not self._p:
+            raise KfftError(f"conv2 config({x},{y}) failed: {last_error(half)}")
+
+    def __call__(self, a, b, out=None):
+        a = _as_input(a, not self.real, self.half)
+        b = _as_input(b, not self.real, self.half)
+        if out is None:
+            out = _empty_like_kind(a, tuple(a.shape), not self.real, self.half)
+        ev = self.L.kfft_eval2_conv_scalar if self.real else self.L.kfft_eval2_conv_cpx
+        check(ev(self._p, _ptr(a), _ptr(b), _ptr(out)), "kfft_eval2_conv", self.half)
+        return out
+
+
+def synchronize(half: bool = False) -> None:
+    check(load(half).kfft_gpu_synchronize(), "kfft_gpu_synchronize", half)

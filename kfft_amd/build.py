@@ -1,0 +1,81 @@
+"""Build recipe of the gfx950 transform engine (hipcc, in-tree).
+
+Produces
+  kfft_amd/lib/libkfft_amd.so       kfft_scalar = double (the reference default)
+  kfft_amd/lib/libkfft_amd_half.so  kfft_scalar = float  (KFFT_HALF_SCALAR)
+Both export the kfft.h C ABI.  Translation units compile in parallel.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "build")
+LIB = os.path.join(HERE, "lib")
+ARCH = os.environ.get("KFFT_AMD_ARCH", "gfx950")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CFLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden",
+    "-DKFFT_EXPORTS", f"-I{os.path.join(ROOT, 'include')}",
+    # packed FP32 ops are kept out of generated code (GPU notes)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+ENGINE = ["kfa_fft_f64.hip", "kfa_fft_f32.hip", "kfa_aux.hip", "kfa_engine.cpp"]
+VARIANTS = {
+    "libkfft_amd.so": [],
+    "libkfft_amd_half.so": ["-DKFFT_HALF_SCALAR"],
+}
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        raise RuntimeError("command failed:\n" + " ".join(cmd) + "\n" + out)
+    return out
+
+
+def _compile(src, obj, extra=()):
+    lang = ["-x", "hip"] if src.endswith(".cpp") else []
+    _run([HIPCC, *CFLAGS, *extra, *lang, "-c", os.path.join(CSRC, src), "-o", obj])
+    return obj
+
+
+def build(jobs: int | None = None, verbose: bool = False) -> list[str]:
+    os.makedirs(OBJ, exist_ok=True)
+    os.makedirs(LIB, exist_ok=True)
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    tasks = []
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        eng = {s: ex.submit(_compile, s, os.path.join(OBJ, s.rsplit(".", 1)[0] + ".o")) for s in ENGINE}
+        api = {
+            name: ex.submit(_compile, "kfa_api.cpp",
+                            os.path.join(OBJ, f"kfa_api_{name.split('.')[0]}.o"), flags)
+            for name, flags in VARIANTS.items()
+        }
+        eng_objs = [f.result() for f in eng.values()]
+        outs = []
+        for name, fut in api.items():
+            so = os.path.join(LIB, name)
+            tmp = so + ".tmp"
+            _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *eng_objs,
+                  fut.result()])
+            os.replace(tmp, so)
+            outs.append(so)
+            if verbose:
+                print("built", so)
+    return outs
+
+
+if __name__ == "__main__":
+    for p in build(verbose=True):
+        print(p)
+    sys.exit(0)

@@ -1,0 +1,1350 @@
+// C-ABI of the MI355X transform engine: the reference's kfft.h entry points
+// (complex, scalar, 2D, conv, conv2d, service, pool) plus the additive
+// batched / asynchronous / device-memory entry points of kfft_gpu.h.
+//
+// Compiled twice: once with kfft_scalar = double (libkfft_amd.so) and once
+// with KFFT_HALF_SCALAR, kfft_scalar = float (libkfft_amd_half.so), exactly as
+// the reference selects its precision at build time (incs/kfft_types.h:21-25).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <new>
+
+#include "../../include/kfft.h"
+#include "kfa_engine.h"
+#include "kfa_kernels.h"
+
+using namespace kfa;
+
+static const int PREC = (int)sizeof(kfft_scalar);
+static const size_t CPXB = sizeof(kfft_cpx);
+
+// ============================================================ GPU objects
+// Device-side state of a plan.  Objects are registered with the pool that
+// holds the plan and destroyed when that pool is freed or cleared.
+struct GpuObj {
+    GpuObj* next = nullptr;
+    std::mutex mu;   // one evaluation of a plan at a time (scratch is shared)
+    DevBuf scratch;  // engine scratch
+    DevBuf stage_in, stage_in2, stage_out; // host-pointer staging
+    DevBuf work_a, work_b; // plan-level work buffers (convolution, 2D)
+    virtual ~GpuObj() = default;
+};
+
+struct CpxObj : GpuObj {
+    std::shared_ptr<Plan1D> p;
+};
+
+struct SclrObj : GpuObj {
+    uint32_t n = 0;
+    std::shared_ptr<Plan1D> p; // length n/2 (even) or n (odd)
+    DevBuf st;                 // super twiddles of the plan's direction (even n)
+};
+
+static std::mutex g_pool_mu;
+
+static void pool_register(kfft_pool_t* A, GpuObj* o) {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    o->next = (GpuObj*)A->gpu;
+    A->gpu = o;
+}
+
+static void pool_release_gpu(kfft_pool_t* A) {
+    GpuObj* o;
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        o = (GpuObj*)A->gpu;
+        A->gpu = nullptr;
+    }
+    while (o) {
+        GpuObj* n = o->next;
+        delete o;
+        o = n;
+    }
+}
+
+// ================================================================= pool
+// Bump allocator with the reference's semantics (kfft_pool.c:13-96): `size`
+// usable bytes, allocations fail once they would pass the end, clear zeroes
+// the area and rewinds.
+extern "C" {
+
+KFFT_API kfft_pool_t* kfft_pool_create(const size_t size) {
+    const size_t total = sizeof(kfft_pool_t) + size;
+    kfft_pool_t* A = (kfft_pool_t*)calloc(1, total);
+    if (!A)
+        return nullptr;
+    A->allocated = total;
+    A->align = 0;
+    A->head = A->area;
+    A->tail = A->area + size;
+    A->cur = A->head;
+    A->gpu = nullptr;
+    return A;
+}
+
+KFFT_API void* kfft_pool_alloc(kfft_pool_t* A, const size_t nmem) {
+    if (!A || nmem == 0)
+        return nullptr;
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    if (A->cur + nmem <= A->tail) {
+        void* r = A->cur;
+        A->cur += nmem;
+        return r;
+    }
+    return nullptr;
+}
+
+KFFT_API size_t kfft_pool_empty(const kfft_pool_t* A) { return A ? (size_t)(A->tail - A->cur) : 0; }
+
+KFFT_API void kfft_pool_zmem(const kfft_pool_t* A, void* ptr, const size_t size) {
+    if (A && ptr && size > 0 && (uint8_t*)ptr + size <= A->tail)
+        memset(ptr, 0, size);
+}
+
+KFFT_API void kfft_pool_clear(kfft_pool_t* A) {
+    if (!A)
+        return;
+    pool_release_gpu(A);
+    kfft_pool_zmem(A, A->area, (size_t)(A->tail - A->head));
+    A->cur = A->head;
+}
+
+KFFT_API void kfft_pool_free(kfft_pool_t* A) {
+    if (!A)
+        return;
+    pool_release_gpu(A);
+    free(A);
+}
+
+} // extern "C"
+
+// Plan creation protocol of the reference (incs/kfft_algo.h:15-51):
+//  lenmem == NULL: allocate in A (cleared first under RENEW) or in a new pool
+//  lenmem != NULL: create only when A is given and *lenmem >= need (the plan
+//                  object then takes *lenmem bytes); else report need in *lenmem
+template <typename T>
+static T* plan_prepare(uint32_t flags, size_t need, kfft_pool_t* A, size_t* lenmem) {
+    T* plan = nullptr;
+    kfft_pool_t* mm = nullptr;
+    if (lenmem == nullptr) {
+        mm = A ? A : kfft_pool_create(need);
+        if (mm) {
+            if (A && (flags & KFFT_FLAG_RENEW))
+                kfft_pool_clear(mm);
+            plan = (T*)kfft_pool_alloc(mm, sizeof(T));
+        }
+        if (plan == nullptr && A == nullptr)
+            kfft_pool_free(mm);
+    } else {
+        if (A && *lenmem >= need) {
+            mm = A;
+            if (flags & KFFT_FLAG_RENEW)
+                kfft_pool_clear(mm);
+            plan = (T*)kfft_pool_alloc(mm, *lenmem);
+        }
+        if (plan == nullptr)
+            *lenmem = need;
+    }
+    if (plan) {
+        memset((void*)plan, 0, sizeof(T));
+        plan->object.mmgr = mm;
+    }
+    return plan;
+}
+
+template <typename T>
+static void plan_terminate(T* plan, kfft_pool_t* A) {
+    if (A == nullptr)
+        kfft_pool_free(KFFT_PLAN_MMGR(plan));
+}
+
+// ======================================================== pointer handling
+static bool is_device_ptr(const void* p) {
+    if (!p)
+        return false;
+    hipPointerAttribute_t at;
+    memset(&at, 0, sizeof(at));
+    hipError_t e = hipPointerGetAttributes(&at, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
+}
+
+static kfft_return_t gpu_fail(const char* what) {
+    if (what)
+        set_error(std::string(what) + ": " + last_error());
+    return KFFT_RET_GPU_FAIL;
+}
+
+// Maps caller buffers to device buffers for one call, staging host memory.
+struct Staged {
+    const void* dev = nullptr;
+    void* host_out = nullptr;
+    size_t bytes = 0;
+    bool staged = false;
+};
+
+static bool stage_in(const void* user, size_t bytes, DevBuf& buf, Staged& s, hipStream_t st) {
+    s.bytes = bytes;
+    if (is_device_ptr(user)) {
+        s.dev = user;
+        return true;
+    }
+    if (!buf.ensure(bytes))
+        return false;
+    if (!hip_ok(hipMemcpyAsync(buf.p, user, bytes, hipMemcpyHostToDevice, st), "stage in"))
+        return false;
+    s.dev = buf.p;
+    s.staged = true;
+    return true;
+}
+
+static bool stage_out(void* user, size_t bytes, DevBuf& buf, Staged& s) {
+    s.bytes = bytes;
+    if (is_device_ptr(user)) {
+        s.dev = user;
+        return true;
+    }
+    if (!buf.ensure(bytes))
+        return false;
+    s.dev = buf.p;
+    s.host_out = user;
+    s.staged = true;
+    return true;
+}
+
+static bool finish_out(const Staged& s, hipStream_t st) {
+    if (s.host_out &&
+        !hip_ok(hipMemcpyAsync(s.host_out, s.dev, s.bytes, hipMemcpyDeviceToHost, st), "stage out"))
+        return false;
+    return true;
+}
+
+// ======================================================= complex 1D helpers
+static size_t cpx_plan_bytes(uint32_t n) { return sizeof(kfft_plan_cpx) + CPXB * n; }
+
+// Reference factor list (kfft_cpx.c:103-143) for the public `factors` field;
+// the engine plans its own radices.
+static void fill_factors(kfft_plan_cpx* P) {
+    uint32_t p = 4, n = P->nfft;
+    uint32_t lim = (uint32_t)std::sqrt((double)P->nfft);
+    P->fac_count = 0;
+    P->prm_count = 0;
+    if (n == 0)
+        return;
+    do {
+        while (n % p) {
+            if (p == 4)
+                p = 2;
+            else if (p == 2)
+                p = 3;
+            else
+                p += 2;
+            if (p > lim)
+                p = n;
+        }
+        if (P->level < KFFT_PLAN_LEVEL && p > KFFT_BFLY_LEVEL && p >= KFFT_RADER_LIMIT &&
+            P->prm_count < MAX_ROOTS)
+            P->primes[P->prm_count++].prime = p;
+        n /= p;
+        if (P->fac_count < MAX_FACTORS) {
+            P->factors[2 * P->fac_count] = p;
+            P->factors[2 * P->fac_count + 1] = n;
+            P->fac_count++;
+        }
+    } while (n > 1);
+}
+
+static void host_twiddles(kfft_cpx* tw, uint32_t n, bool inverse) {
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    for (uint32_t i = 0; i < n; ++i) {
+        long double a = two_pi * (long double)i / (long double)n;
+        tw[i].r = (kfft_scalar)cosl(a);
+        tw[i].i = (kfft_scalar)(inverse ? sinl(a) : -sinl(a));
+    }
+}
+
+static double cpx_scale(const kfft_plan_cpx* P) {
+    if ((P->flags & KFFT_FLAG_INVERSE) && !(P->flags & KFFT_FLAG_DISABLE_NORM))
+        return 1.0 / (double)P->nfft;
+    return 1.0;
+}
+
+static Layout contiguous(const void* p, int64_t dist, uint32_t flags = 0) {
+    Layout l;
+    l.ptr = (void*)p;
+    l.dist_hi = dist;
+    l.dist_lo = 0;
+    l.stride = 1;
+    l.C = 1;
+    l.flags = flags;
+    return l;
+}
+
+// Batched complex transform on device pointers, on stream s, no sync.
+static kfft_return_t cpx_run_dev(kfft_plan_cpx* P, const void* din, void* dout, uint32_t batch,
+                                 hipStream_t s) {
+    CpxObj* g = (CpxObj*)P->gpu;
+    if (!g || !g->p)
+        return KFFT_RET_BADARGUMENTS;
+    const Plan1D& pl = *g->p;
+    if (!g->scratch.ensure(pl.scratch_bytes(batch)))
+        return gpu_fail("scratch");
+    hipError_t e =
+        pl.execute(contiguous(din, P->nfft), contiguous(dout, P->nfft), batch,
+                   (P->flags & KFFT_FLAG_INVERSE) != 0, cpx_scale(P), g->scratch.p, s);
+    return hip_ok(e, "cpx execute") ? KFFT_RET_SUCCESS : KFFT_RET_GPU_FAIL;
+}
+
+static kfft_return_t cpx_run(kfft_plan_cpx* P, const kfft_cpx* fin, kfft_cpx* fout,
+                             uint32_t batch) {
+    if (!P || !fin || !fout)
+        return KFFT_RET_BADARGUMENTS;
+    CpxObj* g = (CpxObj*)P->gpu;
+    if (!g)
+        return KFFT_RET_BADARGUMENTS;
+    std::lock_guard<std::mutex> lk(g->mu);
+    hipStream_t s = default_stream();
+    const size_t bytes = (size_t)P->nfft * CPXB * batch;
+    Staged si, so;
+    if (!stage_in(fin, bytes, g->stage_in, si, s) || !stage_out(fout, bytes, g->stage_out, so))
+        return gpu_fail("staging");
+    kfft_return_t r = cpx_run_dev(P, si.dev, (void*)so.dev, batch, s);
+    if (r != KFFT_RET_SUCCESS)
+        return r;
+    if (!finish_out(so, s) || !hip_ok(hipStreamSynchronize(s), "sync"))
+        return KFFT_RET_GPU_FAIL;
+    return KFFT_RET_SUCCESS;
+}
+
+static kfft_plan_cpx* config_cpx(uint32_t nfft, uint32_t flags, kfft_pool_t* A, size_t* lenmem) {
+    const size_t need = cpx_plan_bytes(nfft);
+    kfft_plan_cpx* P = plan_prepare<kfft_plan_cpx>(flags, need, A, lenmem);
+    if (!P)
+        return nullptr;
+    kfft_pool_t* mm = P->object.mmgr;
+    P->nfft = nfft;
+    P->flags = flags;
+    P->level = 0;
+    fill_factors(P);
+    P->twiddles = (kfft_cpx*)kfft_pool_alloc(mm, CPXB * nfft);
+    if (!P->twiddles) {
+        plan_terminate(P, A);
+        return nullptr;
+    }
+    host_twiddles(P->twiddles, nfft, (flags & KFFT_FLAG_INVERSE) != 0);
+    CpxObj* g = new (std::nothrow) CpxObj();
+    if (!g) {
+        plan_terminate(P, A);
+        return nullptr;
+    }
+    g->p = get_plan(nfft, PREC);
+    if (!g->p) {
+        delete g;
+        plan_terminate(P, A);
+        return nullptr;
+    }
+    pool_register(mm, g);
+    P->gpu = g;
+    return P;
+}
+
+// ======================================================= scalar 1D helpers
+static size_t sclr_plan_bytes(uint32_t n) {
+    if (n % 2)
+        return sizeof(kfft_plan_sclr) + cpx_plan_bytes(n);
+    return sizeof(kfft_plan_sclr) + cpx_plan_bytes(n / 2) + CPXB * (n / 2);
+}
+
+static uint32_t sclr_out_len(const kfft_plan_sclr* P) {
+    if ((P->nfft % 2) || (P->flags & KFFT_FLAG_EXPAND_SCALAR))
+        return P->nfft;
+    return P->nfft / 2 + 1;
+}
+
+static double sclr_scale(const kfft_plan_sclr* P) {
+    if (P->flags & KFFT_FLAG_DISABLE_NORM)
+        return 1.0;
+    return 1.0 / (double)((P->nfft % 2) ? P->nfft : P->nfft / 2);
+}
+
+// Forward real transform of `batch` rows (input distance n reals) into
+// complex rows of distance out_dist.  expand: full-spectrum mirror.
+static kfft_return_t sclr_fwd_dev(kfft_plan_sclr* P, const void* din, void* dout, uint32_t batch,
+                                  int64_t out_dist, bool expand, hipStream_t s) {
+    SclrObj* g = (SclrObj*)P->gpu;
+    const Plan1D& pl = *g->p;
+    if (!g->scratch.ensure(pl.scratch_bytes(batch)))
+        return gpu_fail("scratch");
+    const uint32_t n = P->nfft;
+    if (n % 2) {
+        hipError_t e = pl.execute(contiguous(din, n, KFA_IO_REAL), contiguous(dout, out_dist),
+                                  batch, false, 1.0, g->scratch.p, s);
+        return hip_ok(e, "scalar fwd") ? KFFT_RET_SUCCESS : KFFT_RET_GPU_FAIL;
+    }
+    const uint32_t h = n / 2;
+    // Z = FFT_h(x viewed as h complex), written where X goes, then rebuilt in place
+    hipError_t e = pl.execute(contiguous(din, h), contiguous(dout, out_dist), batch, false, 1.0,
+                              g->scratch.p, s);
+    if (!hip_ok(e, "scalar fwd fft"))
+        return KFFT_RET_GPU_FAIL;
+    e = launch_real_post(PREC, dout, out_dist, dout, out_dist, 0, 1, 1, h, batch, g->st.p,
+                         expand ? 1 : 0, 0, s);
+    return hip_ok(e, "scalar post") ? KFFT_RET_SUCCESS : KFFT_RET_GPU_FAIL;
+}
+
+// Inverse real transform: complex rows (distance in_dist, first n/2+1 used
+// for even n) to real rows of n.  `pre` is scratch for the rebuilt spectrum.
+static kfft_return_t sclr_inv_dev(kfft_plan_sclr* P, const void* din, int64_t in_dist, void* dout,
+                                  uint32_t batch, hipStream_t s) {
+    SclrObj* g = (SclrObj*)P->gpu;
+    const Plan1D& pl = *g->p;
+    const uint32_t n = P->nfft;
+    const double sc = sclr_scale(P);
+    if (n % 2) {
+        if (!g->scratch.ensure(pl.scratch_bytes(batch)))
+            return gpu_fail("scratch");
+        hipError_t e = pl.execute(contiguous(din, in_dist), contiguous(dout, n, KFA_IO_REAL),
+                                  batch, true, sc, g->scratch.p, s);
+        return hip_ok(e, "scalar inv") ? KFFT_RET_SUCCESS : KFFT_RET_GPU_FAIL;
+    }
+    const uint32_t h = n / 2;
+    const size_t zbytes = ((size_t)batch * h * CPXB + 255) / 256 * 256;
+    if (!g->scratch.ensure(zbytes + pl.scratch_bytes(batch)))
+        return gpu_fail("scratch");
+    void* z = g->scratch.p;
+    void* scr = (char*)g->scratch.p + zbytes;
+    hipError_t e = launch_real_pre(PREC, din, in_dist, 0, 1, 1, z, h, h, batch, g->st.p, 0, s);
+    if (!hip_ok(e, "scalar pre"))
+        return KFFT_RET_GPU_FAIL;
+    e = pl.execute(contiguous(z, h), contiguous(dout, h), batch, true, sc, scr, s);
+    return hip_ok(e, "scalar inv fft") ? KFFT_RET_SUCCESS : KFFT_RET_GPU_FAIL;
+}
+
+static kfft_plan_sclr* config_sclr(uint32_t nfft, uint32_t flags, kfft_pool_t* A, size_t* lenmem) {
+    const size_t need = sclr_plan_bytes(nfft);
+    kfft_plan_sclr* P = plan_prepare<kfft_plan_sclr>(flags, need, A, lenmem);
+    if (!P)
+        return nullptr;
+    kfft_pool_t* mm = P->object.mmgr;
+    P->nfft = nfft;
+    P->flags = flags;
+    const uint32_t blen = (nfft % 2) ? nfft : nfft / 2;
+    P->basis = config_cpx(blen, KFFT_CHECK_FLAGS(flags), mm, nullptr);
+    if (!P->basis) {
+        plan_terminate(P, A);
+        return nullptr;
+    }
+    SclrObj* g = new (std::nothrow) SclrObj();
+    if (!g) {
+        plan_terminate(P, A);
+        return nullptr;
+    }
+    g->n = nfft;
+    g->p = ((CpxObj*)P->basis->gpu)->p;
+    if (nfft % 2 == 0) {
+        const uint32_t h = nfft / 2;
+        P->super_twiddles = (kfft_cpx*)kfft_pool_alloc(mm, CPXB * h);
+        if (!P->super_twiddles || !make_real_twiddles(g->st, h, PREC, (flags & KFFT_FLAG_INVERSE) != 0)) {
+            delete g;
+            plan_terminate(P, A);
+            return nullptr;
+        }
+        const long double pi = 3.141592653589793238462643383279502884L;
+        for (uint32_t i = 0; i < h; ++i) {
+            long double ph = -pi * ((long double)(i + 1) / h + 0.5L);
+            if (flags & KFFT_FLAG_INVERSE)
+                ph = -ph;
+            P->super_twiddles[i].r = (kfft_scalar)cosl(ph);
+            P->super_twiddles[i].i = (kfft_scalar)sinl(ph);
+        }
+    }
+    pool_register(mm, g);
+    P->gpu = g;
+    return P;
+}
+
+static kfft_return_t sclr_fwd(kfft_plan_sclr* P, const kfft_scalar* fin, kfft_cpx* fout,
+                              uint32_t batch) {
+    if (!P || !fin || !fout || !P->gpu)
+        return KFFT_RET_BADARGUMENTS;
+    if (P->basis->flags & KFFT_FLAG_INVERSE)
+        return KFFT_RET_IMPROPER_PLAN;
+    SclrObj* g = (SclrObj*)P->gpu;
+    std::lock_guard<std::mutex> lk(g->mu);
+    hipStream_t s = default_stream();
+    const uint32_t ol = sclr_out_len(P);
+    Staged si, so;
+    if (!stage_in(fin, (size_t)P->nfft * sizeof(kfft_scalar) * batch, g->stage_in, si, s) ||
+        !stage_out(fout, (size_t)ol * CPXB * batch, g->stage_out, so))
+        return gpu_fail("staging");
+    kfft_return_t r = sclr_fwd_dev(P, si.dev, (void*)so.dev, batch, ol,
+                                   (P->flags & KFFT_FLAG_EXPAND_SCALAR) != 0, s);
+    if (r != KFFT_RET_SUCCESS)
+        return r;
+    if (!finish_out(so, s) || !hip_ok(hipStreamSynchronize(s), "sync"))
+        return KFFT_RET_GPU_FAIL;
+    return KFFT_RET_SUCCESS;
+}
+
+static kfft_return_t sclr_inv(kfft_plan_sclr* P, const kfft_cpx* fin, kfft_scalar* fout,
+                              uint32_t batch) {
+    if (!P || !fin || !fout || !P->gpu)
+        return KFFT_RET_BADARGUMENTS;
+    if (!(P->basis->flags & KFFT_FLAG_INVERSE))
+        return KFFT_RET_IMPROPER_PLAN;
+    SclrObj* g = (SclrObj*)P->gpu;
+    std::lock_guard<std::mutex> lk(g->mu);
+    hipStream_t s = default_stream();
+    const uint32_t il = sclr_out_len(P);
+    Staged si, so;
+    if (!stage_in(fin, (size_t)il * CPXB * batch, g->stage_in, si, s) ||
+        !stage_out(fout, (size_t)P->nfft * sizeof(kfft_scalar) * batch, g->stage_out, so))
+        return gpu_fail("staging");
+    kfft_return_t r = sclr_inv_dev(P, si.dev, il, (void*)so.dev, batch, s);
+    if (r != KFFT_RET_SUCCESS)
+        return r;
+    if (!finish_out(so, s) || !hip_ok(hipStreamSynchronize(s), "sync"))
+        return KFFT_RET_GPU_FAIL;
+    return KFFT_RET_SUCCESS;
+}
+
+// ============================================================== 2D helpers
+// Column transforms of a y-by-x row-major complex matrix (batch `nmat`),
+// in place: strided LDS passes when the column plan allows it, otherwise
+// transpose / rows / transpose through `work`.
+static kfft_return_t cols_cpx_dev(const Plan1D& py, void* data, uint32_t x, uint32_t y,
+                                  uint32_t nmat, bool inverse, double scale, DevBuf& scratch,
+                                  DevBuf& work, hipStream_t s) {
+    if (py.supports_columns()) {
+        if (!scratch.ensure(py.scratch_bytes((uint64_t)x * nmat)))
+            return gpu_fail("scratch");
+        Layout l;
+        l.ptr = data;
+        l.C = x;
+        l.dist_hi = (int64_t)x * y;
+        l.dist_lo = 1;
+        l.stride = x;
+        hipError_t e = py.execute(l, l, (uint64_t)x * nmat, inverse, scale, scratch.p, s);
+        return hip_ok(e, "columns") ? KFFT_RET_SUCCESS : KFFT_RET_GPU_FAIL;
+    }
+    const size_t bytes = (size_t)x * y * nmat * CPXB;
+    if (!work.ensure(bytes) || !scratch.ensure(py.scratch_bytes((uint64_t)x * nmat)))
+        return gpu_fail("work");
+    if (!hip_ok(launch_transpose((int)CPXB, data, work.p, x, y, nmat, s), "transpose"))
+        return KFFT_RET_GPU_FAIL;
+    hipError_t e = py.execute(contiguous(work.p, y), contiguous(work.p, y), (uint64_t)x * nmat,
+                              inverse, scale, scratch.p, s);
+    if (!hip_ok(e, "transposed columns"))
+        return KFFT_RET_GPU_FAIL;
+    return hip_ok(launch_transpose((int)CPXB, work.p, data, y, x, nmat, s), "transpose back")
+               ? KFFT_RET_SUCCESS
+               : KFFT_RET_GPU_FAIL;
+}
+
+struct C2dObj : GpuObj {};
+
+static size_t c2d_plan_bytes(uint32_t x, uint32_t y) {
+    size_t r = sizeof(kfft_plan_c2d) + cpx_plan_bytes(x);
+    if (x != y)
+        r += cpx_plan_bytes(y);
+    return r;
+}
+
+static kfft_return_t c2d_run_dev(kfft_plan_c2d* P, const void* din, void* dout, hipStream_t s) {
+    C2dObj* g = (C2dObj*)P->gpu;
+    const Plan1D& px = *((CpxObj*)P->plan_x->gpu)->p;
+    const Plan1D& py = *((CpxObj*)P->plan_y->gpu)->p;
+    const bool inv = (P->flags & KFFT_FLAG_INVERSE) != 0;
+    const bool norm = inv && !(P->flags & KFFT_FLAG_DISABLE_NORM);
+    if (!g->scratch.ensure(px.scratch_bytes(P->y)))
+        return gpu_fail("scratch");
+    hipError_t e = px.execute(contiguous(din, P->x), contiguous(dout, P->x), P->y, inv,
+                              norm ? 1.0 / P->x : 1.0, g->scratch.p, s);
+    if (!hip_ok(e, "2d rows"))
+        return KFFT_RET_GPU_FAIL;
+    return cols_cpx_dev(py, dout, P->x, P->y, 1, inv, norm ? 1.0 / P->y : 1.0, g->scratch,
+                        g->work_a, s);
+}
+
+static kfft_plan_c2d* config2_cpx(uint32_t x, uint32_t y, uint32_t flags, kfft_pool_t* A,
+                                  size_t* lenmem) {
+    kfft_plan_c2d* P = plan_prepare<kfft_plan_c2d>(flags, c2d_plan_bytes(x, y), A, lenmem);
+    if (!P)
+        return nullptr;
+    kfft_pool_t* mm = P->object.mmgr;
+    P->nfft = x * y;
+    P->x = x;
+    P->y = y;
+    P->flags = flags;
+    P->plan_x = config_cpx(x, KFFT_CHECK_FLAGS(flags), mm, nullptr);
+    P->plan_y = (x == y) ? P->plan_x : config_cpx(y, KFFT_CHECK_FLAGS(flags), mm, nullptr);
+    C2dObj* g = (P->plan_x && P->plan_y) ? new (std::nothrow) C2dObj() : nullptr;
+    if (!g) {
+        plan_terminate(P, A);
+        return nullptr;
+    }
+    pool_register(mm, g);
+    P->gpu = g;
+    return P;
+}
+
+static kfft_return_t c2d_run(kfft_plan_c2d* P, const kfft_cpx* fin, kfft_cpx* fout) {
+    if (!P || !fin || !fout || !P->gpu)
+        return KFFT_RET_BADARGUMENTS;
+    C2dObj* g = (C2dObj*)P->gpu;
+    std::lock_guard<std::mutex> lk(g->mu);
+    hipStream_t s = default_stream();
+    const size_t bytes = (size_t)P->nfft * CPXB;
+    Staged si, so;
+    if (!stage_in(fin, bytes, g->stage_in, si, s) || !stage_out(fout, bytes, g->stage_out, so))
+        return gpu_fail("staging");
+    kfft_return_t r = c2d_run_dev(P, si.dev, (void*)so.dev, s);
+    if (r != KFFT_RET_SUCCESS)
+        return r;
+    if (!finish_out(so, s) || !hip_ok(hipStreamSynchronize(s), "sync"))
+        return KFFT_RET_GPU_FAIL;
+    return KFFT_RET_SUCCESS;
+}
+
+// --- 2D real ---------------------------------------------------------------
+struct S2dObj : GpuObj {};
+
+static size_t s2d_plan_bytes(uint32_t x, uint32_t y, uint32_t flags) {
+    if (flags & KFFT_FLAG_INVERSE)
+        return sizeof(kfft_plan_s2d) + cpx_plan_bytes(x) + sclr_plan_bytes(y);
+    return sizeof(kfft_plan_s2d) + sclr_plan_bytes(x) + cpx_plan_bytes(y);
+}
+
+static kfft_plan_s2d* config2_sclr(uint32_t x, uint32_t y, uint32_t flags, kfft_pool_t* A,
+                                   size_t* lenmem) {
+    kfft_plan_s2d* P = plan_prepare<kfft_plan_s2d>(flags, s2d_plan_bytes(x, y, flags), A, lenmem);
+    if (!P)
+        return nullptr;
+    kfft_pool_t* mm = P->object.mmgr;
+    P->nfft = x * y;
+    P->x = x;
+    P->y = y;
+    P->flags = flags;
+    const uint32_t f = KFFT_CHECK_FLAGS(flags);
+    if (flags & KFFT_FLAG_INVERSE) {
+        P->plan_x = (kfft_object_t*)config_cpx(x, f, mm, nullptr);
+        P->plan_y = (kfft_object_t*)config_sclr(y, f | KFFT_FLAG_EXPAND_SCALAR, mm, nullptr);
+    } else {
+        P->plan_x = (kfft_object_t*)config_sclr(x, f | KFFT_FLAG_EXPAND_SCALAR, mm, nullptr);
+        P->plan_y = (kfft_object_t*)config_cpx(y, f, mm, nullptr);
+    }
+    S2dObj* g = (P->plan_x && P->plan_y) ? new (std::nothrow) S2dObj() : nullptr;
+    if (!g) {
+        plan_terminate(P, A);
+        return nullptr;
+    }
+    pool_register(mm, g);
+    P->gpu = g;
+    return P;
+}
+
+// forward: real rows with full-spectrum rows, then complex columns
+static kfft_return_t s2d_fwd_dev(kfft_plan_s2d* P, const void* din, void* dout, hipStream_t s) {
+    S2dObj* g = (S2dObj*)P->gpu;
+    kfft_plan_sclr* sx = (kfft_plan_sclr*)P->plan_x;
+    kfft_plan_cpx* cy = (kfft_plan_cpx*)P->plan_y;
+    kfft_return_t r = sclr_fwd_dev(sx, din, dout, P->y, P->x, true, s);
+    if (r != KFFT_RET_SUCCESS)
+        return r;
+    return cols_cpx_dev(*((CpxObj*)cy->gpu)->p, dout, P->x, P->y, 1, false, 1.0, g->scratch,
+                        g->work_a, s);
+}
+
+// inverse: complex rows, then real columns (rebuilt half spectrum per column)
+static kfft_return_t s2d_inv_dev(kfft_plan_s2d* P, const void* din, void* dout, hipStream_t s) {
+    S2dObj* g = (S2dObj*)P->gpu;
+    kfft_plan_cpx* cx_ = (kfft_plan_cpx*)P->plan_x;
+    kfft_plan_sclr* sy = (kfft_plan_sclr*)P->plan_y;
+    const Plan1D& px = *((CpxObj*)cx_->gpu)->p;
+    SclrObj* gy = (SclrObj*)sy->gpu;
+    const Plan1D& py = *gy->p;
+    const uint32_t x = P->x, y = P->y;
+    const size_t nb = (size_t)x * y * CPXB;
+    if (!g->work_a.ensure(nb) || !g->scratch.ensure(px.scratch_bytes(y)))
+        return gpu_fail("work");
+    hipError_t e = px.execute(contiguous(din, x), contiguous(g->work_a.p, x), y, true,
+                              cpx_scale(cx_), g->scratch.p, s);
+    if (!hip_ok(e, "2d inverse rows"))
+        return KFFT_RET_GPU_FAIL;
+    const double sc = sclr_scale(sy);
+    if (!py.supports_columns()) {
+        // transpose to column-major rows, real inverse rows, real transpose back
+        if (!g->work_b.ensure(nb))
+            return gpu_fail("work");
+        if (!hip_ok(launch_transpose((int)CPXB, g->work_a.p, g->work_b.p, x, y, 1, s), "T"))
+            return KFFT_RET_GPU_FAIL;
+        kfft_return_t r = sclr_inv_dev(sy, g->work_b.p, y, g->work_a.p, x, s);
+        if (r != KFFT_RET_SUCCESS)
+            return r;
+        return hip_ok(launch_transpose((int)sizeof(kfft_scalar), g->work_a.p, dout, y, x, 1, s),
+                      "T back")
+                   ? KFFT_RET_SUCCESS
+                   : KFFT_RET_GPU_FAIL;
+    }
+    if (y % 2) {
+        if (!g->scratch.ensure(py.scratch_bytes(x)))
+            return gpu_fail("scratch");
+        Layout li, lo;
+        li.ptr = g->work_a.p;
+        li.C = x;
+        li.dist_hi = (int64_t)x * y;
+        li.dist_lo = 1;
+        li.stride = x;
+        lo = li;
+        lo.ptr = dout;
+        lo.flags = KFA_IO_REAL;
+        e = py.execute(li, lo, x, true, sc, g->scratch.p, s);
+        return hip_ok(e, "2d inverse columns") ? KFFT_RET_SUCCESS : KFFT_RET_GPU_FAIL;
+    }
+    const uint32_t h = y / 2;
+    const size_t zb = ((size_t)x * h * CPXB + 255) / 256 * 256;
+    if (!g->scratch.ensure(zb + py.scratch_bytes(x)))
+        return gpu_fail("scratch");
+    void* z = g->scratch.p;
+    e = launch_real_pre(PREC, g->work_a.p, (int64_t)x * y, 1, x, x, z, h, h, x, gy->st.p, 0, s);
+    if (!hip_ok(e, "2d column pre"))
+        return KFFT_RET_GPU_FAIL;
+    Layout lo;
+    lo.ptr = dout;
+    lo.C = x;
+    lo.dist_hi = (int64_t)x * y;
+    lo.dist_lo = 1;
+    lo.stride = x;
+    lo.flags = KFA_IO_REAL_PAIRS;
+    e = py.execute(contiguous(z, h), lo, x, true, sc, (char*)g->scratch.p + zb, s);
+    return hip_ok(e, "2d inverse columns") ? KFFT_RET_SUCCESS : KFFT_RET_GPU_FAIL;
+}
+
+// ======================================================== convolution
+struct ConvObj : GpuObj {};
+
+static void hadamard_dev(void* acc, const void* other, uint64_t n, hipStream_t s) {
+    (void)hip_ok(launch_hadamard(PREC, acc, other, n, s), "hadamard");
+}
+
+// ------------------------------------------------------------------ shift
+// fftshift / ifftshift of one row (kfft_shift.c:4-61): for even sizes the
+// halves swap; for odd sizes the rotation is by floor(n/2) (forward) or
+// ceil(n/2) (inverse).  Host buffers only; device buffers are round-tripped.
+template <typename T>
+static void shift_host(T* buf, uint32_t size, bool inverse) {
+    if (size < 2)
+        return;
+    const uint32_t c = size / 2;
+    // forward: out[k] = in[(k - c) mod size] (roll by floor(size/2)); inverse undoes it
+    const uint32_t rot = inverse ? (size - c) : c;
+    T* tmp = (T*)malloc(sizeof(T) * size);
+    if (!tmp)
+        return;
+    for (uint32_t k = 0; k < size; ++k)
+        tmp[(k + rot) % size] = buf[k];
+    memcpy(buf, tmp, sizeof(T) * size);
+    free(tmp);
+}
+
+template <typename T>
+static void shift_any(T* buf, uint32_t size, bool inverse) {
+    if (!is_device_ptr(buf)) {
+        shift_host(buf, size, inverse);
+        return;
+    }
+    T* h = (T*)malloc(sizeof(T) * size);
+    if (!h)
+        return;
+    (void)hipMemcpy(h, buf, sizeof(T) * size, hipMemcpyDeviceToHost);
+    shift_host(h, size, inverse);
+    (void)hipMemcpy(buf, h, sizeof(T) * size, hipMemcpyHostToDevice);
+    free(h);
+}
+
+
+// 2D shift (ext/2d/kfft_cpx2.c:188-240): rows, then columns.
+template <typename T>
+static void shift2_any(T* buf, uint32_t x, uint32_t y, bool inverse) {
+    const size_t n = (size_t)x * y;
+    const bool dev = is_device_ptr(buf);
+    T* h = dev ? (T*)malloc(sizeof(T) * n) : buf;
+    if (!h)
+        return;
+    if (dev)
+        (void)hipMemcpy(h, buf, sizeof(T) * n, hipMemcpyDeviceToHost);
+    for (uint32_t r = 0; r < y; ++r)
+        shift_host(h + (size_t)r * x, x, inverse);
+    T* col = (T*)malloc(sizeof(T) * y);
+    if (col) {
+        for (uint32_t c = 0; c < x; ++c) {
+            for (uint32_t r = 0; r < y; ++r)
+                col[r] = h[(size_t)r * x + c];
+            shift_host(col, y, inverse);
+            for (uint32_t r = 0; r < y; ++r)
+                h[(size_t)r * x + c] = col[r];
+        }
+        free(col);
+    }
+    if (dev) {
+        (void)hipMemcpy(buf, h, sizeof(T) * n, hipMemcpyHostToDevice);
+        free(h);
+    }
+}
+
+
+// ====================================================================== API
+extern "C" {
+
+// ---------------------------------------------------------------- service
+KFFT_API void kfft_info(kfft_info_t* info) {
+    if (!info)
+        return;
+    info->vmajor = KFFT_MAJOR_VERSION;
+    info->vminor = KFFT_MINOR_VERSION;
+    info->vpatch = KFFT_PATCH_VERSION;
+    info->flags = KFFT_INFO_RADER_ALGO | KFFT_INFO_GPU_HIP;
+#if defined(KFFT_HALF_SCALAR)
+    info->flags |= KFFT_INFO_HALF_SCALAR;
+#endif
+}
+
+KFFT_API uint32_t kfft_next_fast_size(uint32_t n) {
+    for (;; n++) {
+        uint32_t m = n;
+        while (m % 2 == 0)
+            m /= 2;
+        while (m % 3 == 0)
+            m /= 3;
+        while (m % 5 == 0)
+            m /= 5;
+        if (m <= 1)
+            return n;
+    }
+}
+
+KFFT_API void* kfft_malloc(uint32_t sz) { return aligned_alloc(64, ((size_t)sz + 63) / 64 * 64); }
+
+KFFT_API void kfft_free_null(void** mem) {
+    if (!mem)
+        return;
+    free(*mem);
+    *mem = nullptr;
+}
+
+KFFT_API kfft_return_t kfft_cleanup(void* mem) {
+    if (!mem)
+        return KFFT_RET_FREE_NULL;
+    kfft_object_t* M = (kfft_object_t*)mem;
+    if (M->mmgr == nullptr)
+        return KFFT_RET_FREE_NULL;
+    kfft_pool_free(M->mmgr);
+    return KFFT_RET_SUCCESS;
+}
+
+KFFT_API const char* kfft_strerr(const kfft_return_t code) {
+    static const char* msgs[] = {"Success operation result",
+                                 "Internal allocation memory fail",
+                                 "Temporary buffer allocation fail",
+                                 "Invalide free() operation for NULL buffer",
+                                 "Use improper real function for this plan",
+                                 "Use missing argument for configure plan",
+                                 "GPU operation failed (see kfft_gpu_last_error)"};
+    return code < sizeof(msgs) / sizeof(msgs[0]) ? msgs[code] : "Unknown error";
+}
+
+// ---------------------------------------------------------------- complex
+KFFT_API kfft_plan_cpx* kfft_config_cpx(const uint32_t nfft, const uint32_t flags, kfft_pool_t* A,
+                                        size_t* lenmem) {
+    return config_cpx(nfft, flags, A, lenmem);
+}
+
+KFFT_API kfft_return_t kfft_eval_cpx(kfft_plan_cpx* plan, const kfft_cpx* fin, kfft_cpx* fout) {
+    return cpx_run(plan, fin, fout, 1);
+}
+
+// ----------------------------------------------------------------- scalar
+KFFT_API kfft_plan_sclr* kfft_config_scalar(const uint32_t nfft, const uint32_t flags,
+                                            kfft_pool_t* A, size_t* lenmem) {
+    return config_sclr(nfft, flags, A, lenmem);
+}
+KFFT_API kfft_return_t kfft_eval_scalar(kfft_plan_sclr* plan, const kfft_scalar* fin,
+                                        kfft_cpx* fout) {
+    return sclr_fwd(plan, fin, fout, 1);
+}
+KFFT_API kfft_return_t kfft_evali_scalar(kfft_plan_sclr* plan, const kfft_cpx* fin,
+                                         kfft_scalar* fout) {
+    return sclr_inv(plan, fin, fout, 1);
+}
+KFFT_API kfft_return_t kfft_eval_scalar_internal(kfft_plan_sclr* plan, const kfft_scalar* fin,
+                                                 kfft_cpx* fout, kfft_cpx* ftmp) {
+    (void)ftmp;
+    return sclr_fwd(plan, fin, fout, 1);
+}
+KFFT_API kfft_return_t kfft_evali_scalar_internal(kfft_plan_sclr* plan, const kfft_cpx* fin,
+                                                  kfft_scalar* fout, kfft_cpx* ftmp) {
+    (void)ftmp;
+    return sclr_inv(plan, fin, fout, 1);
+}
+
+// --------------------------------------------------------------------- 2D
+KFFT_API kfft_plan_c2d* kfft_config2_cpx(const uint32_t x_size, const uint32_t y_size,
+                                         const uint32_t flags, kfft_pool_t* A, size_t* lenmem) {
+    return config2_cpx(x_size, y_size, flags, A, lenmem);
+}
+KFFT_API kfft_return_t kfft_eval2_cpx(kfft_plan_c2d* cfg, const kfft_cpx* fin, kfft_cpx* fout) {
+    return c2d_run(cfg, fin, fout);
+}
+
+KFFT_API kfft_plan_s2d* kfft_config2_scalar(const uint32_t x_size, const uint32_t y_size,
+                                            const uint32_t flags, kfft_pool_t* A,
+                                            size_t* lenmem) {
+    return config2_sclr(x_size, y_size, flags, A, lenmem);
+}
+
+KFFT_API kfft_return_t kfft_eval2_scalar(kfft_plan_s2d* plan, const kfft_scalar* fin,
+                                         kfft_cpx* fout) {
+    if (!plan || !fin || !fout || !plan->gpu)
+        return KFFT_RET_BADARGUMENTS;
+    if (plan->flags & KFFT_FLAG_INVERSE)
+        return KFFT_RET_IMPROPER_PLAN;
+    S2dObj* g = (S2dObj*)plan->gpu;
+    std::lock_guard<std::mutex> lk(g->mu);
+    hipStream_t s = default_stream();
+    Staged si, so;
+    if (!stage_in(fin, (size_t)plan->nfft * sizeof(kfft_scalar), g->stage_in, si, s) ||
+        !stage_out(fout, (size_t)plan->nfft * CPXB, g->stage_out, so))
+        return gpu_fail("staging");
+    kfft_return_t r = s2d_fwd_dev(plan, si.dev, (void*)so.dev, s);
+    if (r != KFFT_RET_SUCCESS)
+        return r;
+    return (finish_out(so, s) && hip_ok(hipStreamSynchronize(s), "sync")) ? KFFT_RET_SUCCESS
+                                                                          : KFFT_RET_GPU_FAIL;
+}
+
+KFFT_API kfft_return_t kfft_evali2_scalar(kfft_plan_s2d* plan, const kfft_cpx* fin,
+                                          kfft_scalar* fout) {
+    if (!plan || !fin || !fout || !plan->gpu)
+        return KFFT_RET_BADARGUMENTS;
+    if (!(plan->flags & KFFT_FLAG_INVERSE))
+        return KFFT_RET_IMPROPER_PLAN;
+    S2dObj* g = (S2dObj*)plan->gpu;
+    std::lock_guard<std::mutex> lk(g->mu);
+    hipStream_t s = default_stream();
+    Staged si, so;
+    if (!stage_in(fin, (size_t)plan->nfft * CPXB, g->stage_in, si, s) ||
+        !stage_out(fout, (size_t)plan->nfft * sizeof(kfft_scalar), g->stage_out, so))
+        return gpu_fail("staging");
+    kfft_return_t r = s2d_inv_dev(plan, si.dev, (void*)so.dev, s);
+    if (r != KFFT_RET_SUCCESS)
+        return r;
+    return (finish_out(so, s) && hip_ok(hipStreamSynchronize(s), "sync")) ? KFFT_RET_SUCCESS
+                                                                          : KFFT_RET_GPU_FAIL;
+}
+
+// ------------------------------------------------------------ convolution
+// FFT(A), FFT(B) with the forward plan, pointwise product, inverse plan
+// (ext/conv/kfft_cpx_conv.c:75-104); the product is fused into the inverse
+// transform's load where the inverse is a one- or two-pass transform.
+KFFT_API kfft_plan_ccnv* kfft_config_conv_cpx(const uint32_t nfft, const uint32_t flags,
+                                              kfft_pool_t* A, size_t* lenmem) {
+    const size_t need = sizeof(kfft_plan_ccnv) + 2 * cpx_plan_bytes(nfft);
+    kfft_plan_ccnv* P = plan_prepare<kfft_plan_ccnv>(flags, need, A, lenmem);
+    if (!P)
+        return nullptr;
+    kfft_pool_t* mm = P->object.mmgr;
+    P->nfft = nfft;
+    P->flags = flags;
+    const uint32_t f = KFFT_CHECK_FLAGS_CCNV(flags);
+    P->plan_fwd = config_cpx(nfft, f, mm, nullptr);
+    P->plan_inv = config_cpx(nfft, f | KFFT_FLAG_INVERSE, mm, nullptr);
+    ConvObj* g = (P->plan_fwd && P->plan_inv) ? new (std::nothrow) ConvObj() : nullptr;
+    if (!g) {
+        plan_terminate(P, A);
+        return nullptr;
+    }
+    pool_register(mm, g);
+    P->gpu = g;
+    return P;
+}
+
+KFFT_API kfft_return_t kfft_eval_conv_cpx(kfft_plan_ccnv* plan, const kfft_cpx* fin_A,
+                                          const kfft_cpx* fin_B, kfft_cpx* fout) {
+    if (!plan || !fin_A || !fin_B || !fout || !plan->gpu)
+        return KFFT_RET_BADARGUMENTS;
+    ConvObj* g = (ConvObj*)plan->gpu;
+    std::lock_guard<std::mutex> lk(g->mu);
+    hipStream_t s = default_stream();
+    const uint32_t n = plan->nfft;
+    const size_t bytes = (size_t)n * CPXB;
+    Staged sa, sb, so;
+    if (!stage_in(fin_A, bytes, g->stage_in, sa, s) || !stage_in(fin_B, bytes, g->stage_in2, sb, s) ||
+        !stage_out(fout, bytes, g->stage_out, so) || !g->work_a.ensure(bytes) ||
+        !g->work_b.ensure(bytes))
+        return gpu_fail("staging");
+    const Plan1D& pf = *((CpxObj*)plan->plan_fwd->gpu)->p;
+    if (!g->scratch.ensure(pf.scratch_bytes(1)))
+        return gpu_fail("scratch");
+    hipError_t e = pf.execute(contiguous(sa.dev, n), contiguous(g->work_a.p, n), 1, false, 1.0,
+                              g->scratch.p, s);
+    if (e == hipSuccess)
+        e = pf.execute(contiguous(sb.dev, n), contiguous(g->work_b.p, n), 1, false, 1.0,
+                       g->scratch.p, s);
+    if (!hip_ok(e, "conv forward"))
+        return KFFT_RET_GPU_FAIL;
+    const double sc = cpx_scale(plan->plan_inv);
+    if (pf.kind() == Kind::Direct || pf.kind() == Kind::Identity) {
+        Fuse fz;
+        fz.pw_in = g->work_a.p;
+        e = pf.execute(contiguous(g->work_b.p, n), contiguous(so.dev, n), 1, true, sc, g->scratch.p,
+                       s, &fz);
+    } else {
+        hadamard_dev(g->work_b.p, g->work_a.p, n, s);
+        e = pf.execute(contiguous(g->work_b.p, n), contiguous(so.dev, n), 1, true, sc, g->scratch.p,
+                       s);
+    }
+    if (!hip_ok(e, "conv inverse"))
+        return KFFT_RET_GPU_FAIL;
+    return (finish_out(so, s) && hip_ok(hipStreamSynchronize(s), "sync")) ? KFFT_RET_SUCCESS
+                                                                          : KFFT_RET_GPU_FAIL;
+}
+
+KFFT_API kfft_plan_scnv* kfft_config_conv_scalar(const uint32_t nfft, const uint32_t flags,
+                                                 kfft_pool_t* A, size_t* lenmem) {
+    const size_t need = sizeof(kfft_plan_scnv) + 2 * sclr_plan_bytes(nfft);
+    kfft_plan_scnv* P = plan_prepare<kfft_plan_scnv>(flags, need, A, lenmem);
+    if (!P)
+        return nullptr;
+    kfft_pool_t* mm = P->object.mmgr;
+    P->nfft = nfft;
+    P->flags = flags;
+    const uint32_t f = KFFT_CHECK_FLAGS(flags) & ~(uint32_t)KFFT_FLAG_INVERSE;
+    P->plan_fwd = config_sclr(nfft, f, mm, nullptr);
+    P->plan_inv = config_sclr(nfft, f | KFFT_FLAG_INVERSE, mm, nullptr);
+    ConvObj* g = (P->plan_fwd && P->plan_inv) ? new (std::nothrow) ConvObj() : nullptr;
+    if (!g) {
+        plan_terminate(P, A);
+        return nullptr;
+    }
+    pool_register(mm, g);
+    P->gpu = g;
+    return P;
+}
+
+// ext/conv/kfft_scalar_conv.c:74-100: real forward of both inputs, product
+// over the stored spectrum, real inverse (x2 for even n, as the reference).
+KFFT_API kfft_return_t kfft_eval_conv_scalar(kfft_plan_scnv* plan, const kfft_scalar* fin_A,
+                                             const kfft_scalar* fin_B, kfft_scalar* fout) {
+    if (!plan || !fin_A || !fin_B || !fout || !plan->gpu)
+        return KFFT_RET_BADARGUMENTS;
+    ConvObj* g = (ConvObj*)plan->gpu;
+    std::lock_guard<std::mutex> lk(g->mu);
+    hipStream_t s = default_stream();
+    const uint32_t n = plan->nfft, ol = sclr_out_len(plan->plan_fwd);
+    const size_t rb = (size_t)n * sizeof(kfft_scalar), cb = (size_t)ol * CPXB;
+    Staged sa, sb, so;
+    if (!stage_in(fin_A, rb, g->stage_in, sa, s) || !stage_in(fin_B, rb, g->stage_in2, sb, s) ||
+        !stage_out(fout, rb, g->stage_out, so) || !g->work_a.ensure(cb) || !g->work_b.ensure(cb))
+        return gpu_fail("staging");
+    kfft_return_t r = sclr_fwd_dev(plan->plan_fwd, sa.dev, g->work_a.p, 1, ol,
+                                   (plan->plan_fwd->flags & KFFT_FLAG_EXPAND_SCALAR) != 0, s);
+    if (r == KFFT_RET_SUCCESS)
+        r = sclr_fwd_dev(plan->plan_fwd, sb.dev, g->work_b.p, 1, ol,
+                         (plan->plan_fwd->flags & KFFT_FLAG_EXPAND_SCALAR) != 0, s);
+    if (r != KFFT_RET_SUCCESS)
+        return r;
+    hadamard_dev(g->work_b.p, g->work_a.p, ol, s);
+    r = sclr_inv_dev(plan->plan_inv, g->work_b.p, ol, (void*)so.dev, 1, s);
+    if (r != KFFT_RET_SUCCESS)
+        return r;
+    return (finish_out(so, s) && hip_ok(hipStreamSynchronize(s), "sync")) ? KFFT_RET_SUCCESS
+                                                                          : KFFT_RET_GPU_FAIL;
+}
+
+// ext/conv2d/kfft_cpx_conv.c: the reference builds x-by-x plans (plan->y = x
+// at :67), which is a true 2D convolution only for square inputs; this
+// library computes the x-by-y convolution for every shape.
+KFFT_API kfft_plan_c2cnv* kfft_config2_conv_cpx(const uint32_t x, const uint32_t y,
+                                                const uint32_t flags, kfft_pool_t* A,
+                                                size_t* lenmem) {
+    const size_t need = sizeof(kfft_plan_c2cnv) + 2 * c2d_plan_bytes(x, y);
+    kfft_plan_c2cnv* P = plan_prepare<kfft_plan_c2cnv>(flags, need, A, lenmem);
+    if (!P)
+        return nullptr;
+    kfft_pool_t* mm = P->object.mmgr;
+    P->nfft = x * y;
+    P->x = x;
+    P->y = y;
+    P->flags = flags;
+    const uint32_t f = KFFT_CHECK_FLAGS_CCNV(flags);
+    P->plan_fwd = config2_cpx(x, y, f, mm, nullptr);
+    P->plan_inv = config2_cpx(x, y, f | KFFT_FLAG_INVERSE, mm, nullptr);
+    ConvObj* g = (P->plan_fwd && P->plan_inv) ? new (std::nothrow) ConvObj() : nullptr;
+    if (!g) {
+        plan_terminate(P, A);
+        return nullptr;
+    }
+    pool_register(mm, g);
+    P->gpu = g;
+    return P;
+}
+
+KFFT_API kfft_return_t kfft_eval2_conv_cpx(kfft_plan_c2cnv* plan, const kfft_cpx* fin_A,
+                                           const kfft_cpx* fin_B, kfft_cpx* fout) {
+    if (!plan || !fin_A || !fin_B || !fout || !plan->gpu)
+        return KFFT_RET_BADARGUMENTS;
+    ConvObj* g = (ConvObj*)plan->gpu;
+    std::lock_guard<std::mutex> lk(g->mu);
+    hipStream_t s = default_stream();
+    const size_t bytes = (size_t)plan->nfft * CPXB;
+    Staged sa, sb, so;
+    if (!stage_in(fin_A, bytes, g->stage_in, sa, s) || !stage_in(fin_B, bytes, g->stage_in2, sb, s) ||
+        !stage_out(fout, bytes, g->stage_out, so) || !g->work_a.ensure(bytes) ||
+        !g->work_b.ensure(bytes))
+        return gpu_fail("staging");
+    kfft_return_t r = c2d_run_dev(plan->plan_fwd, sa.dev, g->work_a.p, s);
+    if (r == KFFT_RET_SUCCESS)
+        r = c2d_run_dev(plan->plan_fwd, sb.dev, g->work_b.p, s);
+    if (r != KFFT_RET_SUCCESS)
+        return r;
+    hadamard_dev(g->work_b.p, g->work_a.p, plan->nfft, s);
+    r = c2d_run_dev(plan->plan_inv, g->work_b.p, (void*)so.dev, s);
+    if (r != KFFT_RET_SUCCESS)
+        return r;
+    return (finish_out(so, s) && hip_ok(hipStreamSynchronize(s), "sync")) ? KFFT_RET_SUCCESS
+                                                                          : KFFT_RET_GPU_FAIL;
+}
+
+KFFT_API kfft_plan_s2cnv* kfft_config2_conv_scalar(const uint32_t x, const uint32_t y,
+                                                   const uint32_t flags, kfft_pool_t* A,
+                                                   size_t* lenmem) {
+    const uint32_t f = KFFT_CHECK_FLAGS(flags) & ~(uint32_t)KFFT_FLAG_INVERSE;
+    const size_t need = sizeof(kfft_plan_s2cnv) + s2d_plan_bytes(x, y, f) +
+                        s2d_plan_bytes(x, y, f | KFFT_FLAG_INVERSE);
+    kfft_plan_s2cnv* P = plan_prepare<kfft_plan_s2cnv>(flags, need, A, lenmem);
+    if (!P)
+        return nullptr;
+    kfft_pool_t* mm = P->object.mmgr;
+    P->nfft = x * y;
+    P->x = x;
+    P->y = y;
+    P->flags = flags;
+    P->plan_fwd = config2_sclr(x, y, f, mm, nullptr);
+    P->plan_inv = config2_sclr(x, y, f | KFFT_FLAG_INVERSE, mm, nullptr);
+    ConvObj* g = (P->plan_fwd && P->plan_inv) ? new (std::nothrow) ConvObj() : nullptr;
+    if (!g) {
+        plan_terminate(P, A);
+        return nullptr;
+    }
+    pool_register(mm, g);
+    P->gpu = g;
+    return P;
+}
+
+KFFT_API kfft_return_t kfft_eval2_conv_scalar(kfft_plan_s2cnv* plan, const kfft_scalar* fin_A,
+                                              const kfft_scalar* fin_B, kfft_scalar* fout) {
+    if (!plan || !fin_A || !fin_B || !fout || !plan->gpu)
+        return KFFT_RET_BADARGUMENTS;
+    ConvObj* g = (ConvObj*)plan->gpu;
+    std::lock_guard<std::mutex> lk(g->mu);
+    hipStream_t s = default_stream();
+    const size_t rb = (size_t)plan->nfft * sizeof(kfft_scalar), cb = (size_t)plan->nfft * CPXB;
+    Staged sa, sb, so;
+    if (!stage_in(fin_A, rb, g->stage_in, sa, s) || !stage_in(fin_B, rb, g->stage_in2, sb, s) ||
+        !stage_out(fout, rb, g->stage_out, so) || !g->work_a.ensure(cb) || !g->work_b.ensure(cb))
+        return gpu_fail("staging");
+    kfft_return_t r = s2d_fwd_dev(plan->plan_fwd, sa.dev, g->work_a.p, s);
+    if (r == KFFT_RET_SUCCESS)
+        r = s2d_fwd_dev(plan->plan_fwd, sb.dev, g->work_b.p, s);
+    if (r != KFFT_RET_SUCCESS)
+        return r;
+    hadamard_dev(g->work_b.p, g->work_a.p, plan->nfft, s);
+    r = s2d_inv_dev(plan->plan_inv, g->work_b.p, (void*)so.dev, s);
+    if (r != KFFT_RET_SUCCESS)
+        return r;
+    return (finish_out(so, s) && hip_ok(hipStreamSynchronize(s), "sync")) ? KFFT_RET_SUCCESS
+                                                                          : KFFT_RET_GPU_FAIL;
+}
+
+// ------------------------------------------------------------------- math
+KFFT_API void kfft_math_hadamard_cpx(kfft_cpx* Fout, const kfft_cpx* Fin, uint32_t size) {
+    if (is_device_ptr(Fout) && is_device_ptr(Fin)) {
+        hipStream_t s = default_stream();
+        hadamard_dev(Fout, Fin, size, s);
+        (void)hipStreamSynchronize(s);
+        return;
+    }
+    for (uint32_t i = 0; i < size; i++) {
+        const kfft_cpx a = Fout[i], b = Fin[i];
+        Fout[i].r = a.r * b.r - a.i * b.i;
+        Fout[i].i = a.r * b.i + a.i * b.r;
+    }
+}
+
+KFFT_API void kfft_math_hadamard_scalar(kfft_scalar* Fout, const kfft_scalar* Fin, uint32_t size) {
+    if (is_device_ptr(Fout) && is_device_ptr(Fin)) {
+        hipStream_t s = default_stream();
+        (void)hip_ok(launch_hadamard_real(PREC, Fout, Fin, size, s), "hadamard");
+        (void)hipStreamSynchronize(s);
+        return;
+    }
+    for (uint32_t i = 0; i < size; i++)
+        Fout[i] *= Fin[i];
+}
+
+// kfft_math.c:60-78: Fout[i*y + j] = Fin[j*x + i]
+KFFT_API void kfft_math_transpose_cpx(const kfft_cpx* Fin, kfft_cpx* Fout, const uint32_t x,
+                                      const uint32_t y) {
+    if (is_device_ptr(Fin) && is_device_ptr(Fout)) {
+        hipStream_t s = default_stream();
+        (void)hip_ok(launch_transpose((int)CPXB, Fin, Fout, x, y, 1, s), "transpose");
+        (void)hipStreamSynchronize(s);
+        return;
+    }
+    for (uint64_t n = 0; n < (uint64_t)x * y; n++)
+        Fout[n] = Fin[(uint64_t)x * (n % y) + n / y];
+}
+
+KFFT_API void kfft_math_transpose_scalar(const kfft_scalar* Fin, kfft_scalar* Fout,
+                                         const uint32_t x, const uint32_t y) {
+    if (is_device_ptr(Fin) && is_device_ptr(Fout)) {
+        hipStream_t s = default_stream();
+        (void)hip_ok(launch_transpose((int)sizeof(kfft_scalar), Fin, Fout, x, y, 1, s), "transpose");
+        (void)hipStreamSynchronize(s);
+        return;
+    }
+    for (uint64_t n = 0; n < (uint64_t)x * y; n++)
+        Fout[n] = Fin[(uint64_t)x * (n % y) + n / y];
+}
+
+KFFT_API void kfft_shift_cpx(kfft_cpx* buf, const uint32_t size, const bool is_inverse,
+                             kfft_pool_t* mmgr) {
+    (void)mmgr;
+    shift_any(buf, size, is_inverse);
+}
+KFFT_API void kfft_shift_scalar(kfft_scalar* buf, const uint32_t size, const bool is_inverse,
+                                kfft_pool_t* mmgr) {
+    (void)mmgr;
+    shift_any(buf, size, is_inverse);
+}
+
+KFFT_API void kfft_shift2_cpx(kfft_cpx* buf, kfft_cpx* ftmp, const uint32_t sz_x,
+                              const uint32_t sz_y, const bool is_inverse, kfft_pool_t* mmgr) {
+    (void)ftmp;
+    (void)mmgr;
+    shift2_any(buf, sz_x, sz_y, is_inverse);
+}
+KFFT_API void kfft_shift2_scalar(kfft_scalar* buf, kfft_scalar* ftmp, const uint32_t sz_x,
+                                 const uint32_t sz_y, const bool is_inverse, kfft_pool_t* mmgr) {
+    (void)ftmp;
+    (void)mmgr;
+    shift2_any(buf, sz_x, sz_y, is_inverse);
+}
+
+// -------------------------------------------------------------------- GPU
+KFFT_API void* kfft_gpu_malloc(size_t bytes) {
+    if (!runtime_init())
+        return nullptr;
+    void* p = nullptr;
+    return hip_ok(hipMalloc(&p, bytes), "hipMalloc") ? p : nullptr;
+}
+KFFT_API void kfft_gpu_free(void* ptr) {
+    if (ptr)
+        (void)hipFree(ptr);
+}
+KFFT_API kfft_return_t kfft_gpu_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+    return hip_ok(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "h2d") ? KFFT_RET_SUCCESS
+                                                                          : KFFT_RET_GPU_FAIL;
+}
+KFFT_API kfft_return_t kfft_gpu_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+    return hip_ok(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "d2h") ? KFFT_RET_SUCCESS
+                                                                          : KFFT_RET_GPU_FAIL;
+}
+KFFT_API kfft_return_t kfft_gpu_synchronize(void) {
+    if (!runtime_init())
+        return KFFT_RET_GPU_FAIL;
+    return hip_ok(hipStreamSynchronize(default_stream()), "sync") ? KFFT_RET_SUCCESS
+                                                                 : KFFT_RET_GPU_FAIL;
+}
+KFFT_API void* kfft_gpu_stream(void) { return runtime_init() ? (void*)default_stream() : nullptr; }
+KFFT_API const char* kfft_gpu_last_error(void) { return last_error(); }
+
+KFFT_API const char* kfft_gpu_plan_describe(const void* plan) {
+    const kfft_plan_cpx* P = (const kfft_plan_cpx*)plan;
+    if (!P || !P->gpu)
+        return "";
+    CpxObj* g = dynamic_cast<CpxObj*>((GpuObj*)P->gpu);
+    return (g && g->p) ? g->p->describe() : "";
+}
+
+KFFT_API kfft_return_t kfft_eval_cpx_batch(kfft_plan_cpx* plan, const kfft_cpx* fin,
+                                           kfft_cpx* fout, uint32_t batch) {
+    return cpx_run(plan, fin, fout, batch);
+}
+
+KFFT_API kfft_return_t kfft_eval_cpx_batch_async(kfft_plan_cpx* plan, const kfft_cpx* fin,
+                                                 kfft_cpx* fout, uint32_t batch, void* stream) {
+    if (!plan || !plan->gpu || !fin || !fout)
+        return KFFT_RET_BADARGUMENTS;
+    CpxObj* g = (CpxObj*)plan->gpu;
+    std::lock_guard<std::mutex> lk(g->mu);
+    hipStream_t s = stream ? (hipStream_t)stream : default_stream();
+    return cpx_run_dev(plan, fin, fout, batch, s);
+}
+
+KFFT_API uint32_t kfft_scalar_out_len(const kfft_plan_sclr* plan) {
+    return plan ? sclr_out_len(plan) : 0;
+}
+
+KFFT_API kfft_return_t kfft_eval_scalar_batch(kfft_plan_sclr* plan, const kfft_scalar* fin,
+                                              kfft_cpx* fout, uint32_t batch) {
+    return sclr_fwd(plan, fin, fout, batch);
+}
+KFFT_API kfft_return_t kfft_evali_scalar_batch(kfft_plan_sclr* plan, const kfft_cpx* fin,
+                                               kfft_scalar* fout, uint32_t batch) {
+    return sclr_inv(plan, fin, fout, batch);
+}
+KFFT_API kfft_return_t kfft_eval_scalar_batch_async(kfft_plan_sclr* plan, const kfft_scalar* fin,
+                                                    kfft_cpx* fout, uint32_t batch, void* stream) {
+    if (!plan || !plan->gpu || !fin || !fout)
+        return KFFT_RET_BADARGUMENTS;
+    if (plan->basis->flags & KFFT_FLAG_INVERSE)
+        return KFFT_RET_IMPROPER_PLAN;
+    SclrObj* g = (SclrObj*)plan->gpu;
+    std::lock_guard<std::mutex> lk(g->mu);
+    hipStream_t s = stream ? (hipStream_t)stream : default_stream();
+    return sclr_fwd_dev(plan, fin, fout, batch, sclr_out_len(plan),
+                        (plan->flags & KFFT_FLAG_EXPAND_SCALAR) != 0, s);
+}
+KFFT_API kfft_return_t kfft_evali_scalar_batch_async(kfft_plan_sclr* plan, const kfft_cpx* fin,
+                                                     kfft_scalar* fout, uint32_t batch,
+                                                     void* stream) {
+    if (!plan || !plan->gpu || !fin || !fout)
+        return KFFT_RET_BADARGUMENTS;
+    if (!(plan->basis->flags & KFFT_FLAG_INVERSE))
+        return KFFT_RET_IMPROPER_PLAN;
+    SclrObj* g = (SclrObj*)plan->gpu;
+    std::lock_guard<std::mutex> lk(g->mu);
+    hipStream_t s = stream ? (hipStream_t)stream : default_stream();
+    return sclr_inv_dev(plan, fin, sclr_out_len(plan), fout, batch, s);
+}
+
+KFFT_API kfft_return_t kfft_eval2_cpx_async(kfft_plan_c2d* plan, const kfft_cpx* fin,
+                                            kfft_cpx* fout, void* stream) {
+    if (!plan || !plan->gpu || !fin || !fout)
+        return KFFT_RET_BADARGUMENTS;
+    C2dObj* g = (C2dObj*)plan->gpu;
+    std::lock_guard<std::mutex> lk(g->mu);
+    hipStream_t s = stream ? (hipStream_t)stream : default_stream();
+    return c2d_run_dev(plan, fin, fout, s);
+}
+
+} // extern "C"

@@ -1,0 +1,449 @@
+// Auxiliary gfx950 kernels of the transform engine: exact twiddle tables,
+// Bluestein/Rader precompute, real-FFT pre/post passes, tiled transposes,
+// pointwise products and copies; plus the host launchers.
+#include "kfa_device.h"
+#include "kfa_kernels.h"
+
+namespace kfa {
+
+// ---------------------------------------------------------------------------
+// exact-argument twiddles in double: angle = -2*pi*m/M with the octant found
+// from integers and sincospi on the reduced remainder
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ cx<double> twiddle_d(uint64_t m, uint64_t M) {
+    m %= M;
+    const uint64_t u = 8 * m; // angle = (pi/4) * u / M
+    const uint64_t o = u / M, rem = u - o * M;
+    uint64_t quad;
+    double phi; // in units of pi
+    int sg;
+    if ((o & 1) == 0) {
+        quad = o / 2;
+        phi = 0.25 * (double)rem / (double)M;
+        sg = 1;
+    } else {
+        quad = (o + 1) / 2;
+        phi = 0.25 * (double)(M - rem) / (double)M;
+        sg = -1;
+    }
+    double sp, cp;
+    sincospi(phi, &sp, &cp);
+    double ca = 0, sa = 0;
+    switch (quad & 3) {
+    case 0: ca = 1; break;
+    case 1: sa = 1; break;
+    case 2: ca = -1; break;
+    default: sa = -1; break;
+    }
+    const double c = ca * cp - sa * sg * sp, s = sa * cp + ca * sg * sp;
+    return {c, -s}; // exp(-i angle)
+}
+
+template <typename T>
+__global__ void k_stage_twiddles(cx<T>* tw, KfaFftArgs A) {
+    // A.st[s].tw_off / ns / radix describe the table layout; entries L-1 max
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t s = 1; s < A.nstages; ++s) {
+        const uint32_t R = A.st[s].radix, ns = A.st[s].ns, off = A.st[s].tw_off;
+        const uint32_t cnt = ns * (R - 1);
+        if (i >= off && i < off + cnt) {
+            const uint32_t j = i - off, k = j / (R - 1), r = j % (R - 1) + 1;
+            const cx<double> w = twiddle_d((uint64_t)r * k, (uint64_t)ns * R);
+            tw[i] = {(T)w.re, (T)w.im};
+        }
+    }
+}
+
+__global__ void k_twiddle_table(cx<double>* out, uint64_t count, uint64_t step, uint64_t M) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        out[i] = twiddle_d(i * step, M);
+}
+
+// tab[i] = exp(-2 pi i * idx[i] / M)  (Rader b vector)
+__global__ void k_twiddle_indexed(cx<double>* out, const uint32_t* idx, uint64_t count,
+                                  uint64_t M, int conjugate) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) {
+        cx<double> w = twiddle_d(idx[i], M);
+        if (conjugate)
+            w.im = -w.im;
+        out[i] = w;
+    }
+}
+
+// Bluestein chirp c[n] = exp(-i pi n^2 / N) for n < N and its kernel
+// h[m] = conj(c[m]), h[M-m] = conj(c[m]) (1 <= m < N), zero elsewhere.
+__global__ void k_bluestein_tables(cx<double>* chirp, cx<double>* h, uint64_t N, uint64_t M) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) {
+        const uint64_t sq = (i * i) % (2 * N);
+        const cx<double> c = twiddle_d(sq, 2 * N);
+        chirp[i] = c;
+        h[i] = conj(c);
+        if (i > 0)
+            h[M - i] = conj(c);
+    } else if (i < M - N + 1) {
+        h[i] = {0.0, 0.0};
+    }
+}
+
+template <typename T>
+__global__ void k_convert_from_f64(cx<T>* dst, const cx<double>* src, uint64_t n, double scale) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        dst[i] = {(T)(src[i].re * scale), (T)(src[i].im * scale)};
+}
+
+// Real-input post-processing (even length n = 2h): X[k] from Z = FFT_h(x as complex)
+//   X[k]   = ((f1 + t)/2),  X[h-k] = conj((f1 - t)/2)... written out per component
+// with f1 = Z[k] + conj(Z[h-k]), f2 = Z[k] - conj(Z[h-k]), t = f2 * st[k-1].
+// Optional full-spectrum mirror X[n-k] = conj(X[k]).  General strided layout.
+template <typename T>
+__global__ void k_real_post(const cx<T>* Z, int64_t z_dist, cx<T>* X, int64_t x_dist_hi,
+                            int64_t x_dist_lo, int64_t x_stride, uint32_t C, uint32_t h,
+                            uint64_t batch, const cx<T>* st, int expand, int conj_out) {
+    const uint32_t kk = blockIdx.x * blockDim.x + threadIdx.x; // 0..h/2
+    const uint64_t b = blockIdx.y + (uint64_t)gridDim.y * blockIdx.z;
+    if (b >= batch || kk > h / 2)
+        return;
+    const cx<T>* z = Z + (int64_t)b * z_dist;
+    const uint64_t bh = b / C, bl = b - bh * C;
+    cx<T>* x = X + (int64_t)bh * x_dist_hi + (int64_t)bl * x_dist_lo;
+    const uint32_t n = 2 * h;
+    auto put = [&](uint32_t k, cx<T> v) {
+        if (conj_out)
+            v.im = -v.im;
+        x[(int64_t)k * x_stride] = v;
+    };
+    if (kk == 0) {
+        const cx<T> dc = z[0];
+        put(0, {dc.re + dc.im, T(0)});
+        put(h, {dc.re - dc.im, T(0)});
+        return;
+    }
+    const uint32_t k = kk;
+    const cx<T> a = z[k], bz = z[h - k];
+    const cx<T> bb = {bz.re, -bz.im};
+    const cx<T> f1 = a + bb, f2 = a - bb;
+    const cx<T> t = cmul(f2, st[k - 1]);
+    const cx<T> xk = {(f1.re + t.re) * T(0.5), (f1.im + t.im) * T(0.5)};
+    const cx<T> xh = {(f1.re - t.re) * T(0.5), (t.im - f1.im) * T(0.5)};
+    put(k, xk);
+    put(h - k, xh);
+    if (expand) {
+        put(n - k, conj(xk));
+        put(n - (h - k), conj(xh));
+    }
+}
+
+// Real-output pre-processing (even n = 2h): Z[k] from the half spectrum X[0..h]
+// (the reference's rebuild_inverse, which yields 2*Z).  General strided input.
+template <typename T>
+__global__ void k_real_pre(const cx<T>* X, int64_t x_dist_hi, int64_t x_dist_lo, int64_t x_stride,
+                           uint32_t C, cx<T>* Z, int64_t z_dist, uint32_t h, uint64_t batch,
+                           const cx<T>* st, int conj_in) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t b = blockIdx.y + (uint64_t)gridDim.y * blockIdx.z;
+    if (b >= batch || k > h / 2)
+        return;
+    const uint64_t bh = b / C, bl = b - bh * C;
+    const cx<T>* x = X + (int64_t)bh * x_dist_hi + (int64_t)bl * x_dist_lo;
+    cx<T>* z = Z + (int64_t)b * z_dist;
+    auto get = [&](uint32_t i) {
+        cx<T> v = x[(int64_t)i * x_stride];
+        if (conj_in)
+            v.im = -v.im;
+        return v;
+    };
+    if (k == 0) {
+        const cx<T> x0 = get(0), xh = get(h);
+        z[0] = {x0.re + xh.re, x0.re - xh.re};
+        return;
+    }
+    const cx<T> fk = get(k), fr = get(h - k);
+    const cx<T> fnkc = {fr.re, -fr.im};
+    const cx<T> fek = fk + fnkc, tmp = fk - fnkc;
+    const cx<T> fok = cmul(tmp, st[k - 1]);
+    z[k] = fek + fok;
+    const cx<T> w = fek - fok;
+    z[h - k] = {w.re, -w.im};
+}
+
+// out[i*y + j] = in[j*x + i]  (the reference's transpose, kfft_math.c:60-78),
+// through 32x32 LDS tiles; batched over z.
+template <typename T>
+__global__ void k_transpose(const T* in, T* out, uint32_t x, uint32_t y, uint64_t mat) {
+    __shared__ T tile[32][33];
+    const uint64_t zb = (uint64_t)blockIdx.z * mat;
+    const uint32_t i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+    for (uint32_t r = threadIdx.y; r < 32; r += blockDim.y) {
+        const uint32_t j = j0 + r, i = i0 + threadIdx.x;
+        if (i < x && j < y)
+            tile[r][threadIdx.x] = in[zb + (uint64_t)j * x + i];
+    }
+    __syncthreads();
+    for (uint32_t r = threadIdx.y; r < 32; r += blockDim.y) {
+        const uint32_t i = i0 + r, j = j0 + threadIdx.x;
+        if (i < x && j < y)
+            out[zb + (uint64_t)i * y + j] = tile[threadIdx.x][r];
+    }
+}
+
+template <typename T>
+__global__ void k_hadamard(cx<T>* acc, const cx<T>* other, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        acc[i] = cmul(acc[i], other[i]);
+}
+
+template <typename T>
+__global__ void k_hadamard_real(T* acc, const T* other, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        acc[i] *= other[i];
+}
+
+// generic strided complex copy with optional conjugation and scaling
+template <typename T>
+__global__ void k_copy(const cx<T>* in, cx<T>* out, uint64_t n, int conj_, double scale) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        cx<T> v = in[i];
+        if (conj_)
+            v.im = -v.im;
+        out[i] = {(T)(v.re * (T)scale), (T)(v.im * (T)scale)};
+    }
+}
+
+// out[b] = x_b[0] for every instance b (the Rader x[0] values), so that passes
+// writing the output in place never race with readers of x[0].
+template <typename T>
+__global__ void k_gather_first(const T* in, int64_t dist_hi, int64_t dist_lo, uint32_t C, T* out,
+                               uint64_t batch, int comps) {
+    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch)
+        return;
+    const uint64_t hi = b / C, lo = b - hi * C;
+    const int64_t off = (int64_t)hi * dist_hi + (int64_t)lo * dist_lo;
+    for (int c = 0; c < comps; ++c)
+        out[b * comps + c] = in[off * comps + c];
+}
+
+// Rader DC term: X[0] = scale * (x[0] + conj?(A[0])), A = FFT_{p-1} of the
+// gathered input (kept in scratch), written through the output layout.
+template <typename T>
+__global__ void k_rader_dc(const T* x0, int x0_real, const cx<T>* A, uint32_t L, T* out,
+                           int64_t dh, int64_t dl, uint32_t C, int real_out, int conj_,
+                           double scale, uint64_t batch) {
+    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch)
+        return;
+    cx<T> a = A[b * L];
+    if (conj_)
+        a.im = -a.im;
+    const cx<T> xv = x0_real ? cx<T>{x0[b], T(0)} : cx<T>{x0[2 * b], x0[2 * b + 1]};
+    cx<T> d = xv + a;
+    d.re *= (T)scale;
+    d.im *= (T)scale;
+    const uint64_t hi = b / C, lo = b - hi * C;
+    const int64_t off = (int64_t)hi * dh + (int64_t)lo * dl;
+    if (real_out)
+        out[off] = d.re;
+    else
+        ((cx<T>*)out)[off] = d;
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+static inline unsigned blocks_for(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+hipError_t launch_fft(int prec, int maxe, const KfaFftArgs& a, uint32_t lds_bytes, hipStream_t s) {
+    return prec == 8 ? launch_fft_f64(maxe, a, lds_bytes, s) : launch_fft_f32(maxe, a, lds_bytes, s);
+}
+
+hipError_t launch_stage_twiddles(int prec, void* tw, const KfaFftArgs& a, hipStream_t s) {
+    if (a.L < 2 || a.nstages < 2)
+        return hipSuccess;
+    const unsigned bs = 256;
+    if (prec == 8)
+        hipLaunchKernelGGL(k_stage_twiddles<double>, dim3(blocks_for(a.L, bs)), dim3(bs), 0, s,
+                           (cx<double>*)tw, a);
+    else
+        hipLaunchKernelGGL(k_stage_twiddles<float>, dim3(blocks_for(a.L, bs)), dim3(bs), 0, s,
+                           (cx<float>*)tw, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_twiddle_table(void* out_d, uint64_t count, uint64_t step, uint64_t M,
+                                hipStream_t s) {
+    if (!count)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_twiddle_table, dim3(blocks_for(count, 256)), dim3(256), 0, s,
+                       (cx<double>*)out_d, count, step, M);
+    return hipGetLastError();
+}
+
+hipError_t launch_twiddle_indexed(void* out_d, const uint32_t* idx, uint64_t count, uint64_t M,
+                                  int conjugate, hipStream_t s) {
+    if (!count)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_twiddle_indexed, dim3(blocks_for(count, 256)), dim3(256), 0, s,
+                       (cx<double>*)out_d, idx, count, M, conjugate);
+    return hipGetLastError();
+}
+
+hipError_t launch_bluestein_tables(void* chirp_d, void* h_d, uint64_t N, uint64_t M,
+                                   hipStream_t s) {
+    const uint64_t cnt = (M - N + 1 > N) ? (M - N + 1) : N;
+    hipLaunchKernelGGL(k_bluestein_tables, dim3(blocks_for(cnt, 256)), dim3(256), 0, s,
+                       (cx<double>*)chirp_d, (cx<double>*)h_d, N, M);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert_from_f64(int prec, void* dst, const void* src_d, uint64_t n,
+                                   double scale, hipStream_t s) {
+    if (!n)
+        return hipSuccess;
+    if (prec == 8)
+        hipLaunchKernelGGL(k_convert_from_f64<double>, dim3(blocks_for(n, 256)), dim3(256), 0, s,
+                           (cx<double>*)dst, (const cx<double>*)src_d, n, scale);
+    else
+        hipLaunchKernelGGL(k_convert_from_f64<float>, dim3(blocks_for(n, 256)), dim3(256), 0, s,
+                           (cx<float>*)dst, (const cx<double>*)src_d, n, scale);
+    return hipGetLastError();
+}
+
+static dim3 batch_grid(uint32_t gx, uint64_t batch) {
+    const uint64_t y = batch < 65535 ? batch : 65535;
+    const uint64_t z = (batch + y - 1) / y;
+    return dim3(gx, (unsigned)y, (unsigned)z);
+}
+
+hipError_t launch_real_post(int prec, const void* Z, int64_t z_dist, void* X, int64_t xdh,
+                            int64_t xdl, int64_t xs, uint32_t C, uint32_t h, uint64_t batch,
+                            const void* st, int expand, int conj_out, hipStream_t s) {
+    if (!batch)
+        return hipSuccess;
+    const unsigned bs = 128;
+    const dim3 g = batch_grid(blocks_for(h / 2 + 1, bs), batch);
+    if (prec == 8)
+        hipLaunchKernelGGL(k_real_post<double>, g, dim3(bs), 0, s, (const cx<double>*)Z, z_dist,
+                           (cx<double>*)X, xdh, xdl, xs, C, h, batch, (const cx<double>*)st,
+                           expand, conj_out);
+    else
+        hipLaunchKernelGGL(k_real_post<float>, g, dim3(bs), 0, s, (const cx<float>*)Z, z_dist,
+                           (cx<float>*)X, xdh, xdl, xs, C, h, batch, (const cx<float>*)st, expand,
+                           conj_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_real_pre(int prec, const void* X, int64_t xdh, int64_t xdl, int64_t xs,
+                           uint32_t C, void* Z, int64_t z_dist, uint32_t h, uint64_t batch,
+                           const void* st, int conj_in, hipStream_t s) {
+    if (!batch)
+        return hipSuccess;
+    const unsigned bs = 128;
+    const dim3 g = batch_grid(blocks_for(h / 2 + 1, bs), batch);
+    if (prec == 8)
+        hipLaunchKernelGGL(k_real_pre<double>, g, dim3(bs), 0, s, (const cx<double>*)X, xdh, xdl,
+                           xs, C, (cx<double>*)Z, z_dist, h, batch, (const cx<double>*)st,
+                           conj_in);
+    else
+        hipLaunchKernelGGL(k_real_pre<float>, g, dim3(bs), 0, s, (const cx<float>*)X, xdh, xdl, xs,
+                           C, (cx<float>*)Z, z_dist, h, batch, (const cx<float>*)st, conj_in);
+    return hipGetLastError();
+}
+
+hipError_t launch_transpose(int elem_bytes, const void* in, void* out, uint32_t x, uint32_t y,
+                            uint32_t nmat, hipStream_t s) {
+    const dim3 g((x + 31) / 32, (y + 31) / 32, nmat), b(32, 8);
+    const uint64_t mat = (uint64_t)x * y;
+    switch (elem_bytes) {
+    case 4:
+        hipLaunchKernelGGL(k_transpose<float>, g, b, 0, s, (const float*)in, (float*)out, x, y,
+                           mat);
+        break;
+    case 8:
+        hipLaunchKernelGGL(k_transpose<double>, g, b, 0, s, (const double*)in, (double*)out, x, y,
+                           mat);
+        break;
+    default:
+        hipLaunchKernelGGL(k_transpose<cx<double>>, g, b, 0, s, (const cx<double>*)in,
+                           (cx<double>*)out, x, y, mat);
+        break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_hadamard(int prec, void* acc, const void* other, uint64_t n, hipStream_t s) {
+    if (!n)
+        return hipSuccess;
+    if (prec == 8)
+        hipLaunchKernelGGL(k_hadamard<double>, dim3(blocks_for(n, 256)), dim3(256), 0, s,
+                           (cx<double>*)acc, (const cx<double>*)other, n);
+    else
+        hipLaunchKernelGGL(k_hadamard<float>, dim3(blocks_for(n, 256)), dim3(256), 0, s,
+                           (cx<float>*)acc, (const cx<float>*)other, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_hadamard_real(int prec, void* acc, const void* other, uint64_t n,
+                                hipStream_t s) {
+    if (!n)
+        return hipSuccess;
+    if (prec == 8)
+        hipLaunchKernelGGL(k_hadamard_real<double>, dim3(blocks_for(n, 256)), dim3(256), 0, s,
+                           (double*)acc, (const double*)other, n);
+    else
+        hipLaunchKernelGGL(k_hadamard_real<float>, dim3(blocks_for(n, 256)), dim3(256), 0, s,
+                           (float*)acc, (const float*)other, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(int prec, const void* in, void* out, uint64_t n, int conj_, double scale,
+                       hipStream_t s) {
+    if (!n)
+        return hipSuccess;
+    if (prec == 8)
+        hipLaunchKernelGGL(k_copy<double>, dim3(blocks_for(n, 256)), dim3(256), 0, s,
+                           (const cx<double>*)in, (cx<double>*)out, n, conj_, scale);
+    else
+        hipLaunchKernelGGL(k_copy<float>, dim3(blocks_for(n, 256)), dim3(256), 0, s,
+                           (const cx<float>*)in, (cx<float>*)out, n, conj_, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_first(int prec, int real_in, const void* in, int64_t dist_hi,
+                               int64_t dist_lo, uint32_t C, void* out, uint64_t batch,
+                               hipStream_t s) {
+    if (!batch)
+        return hipSuccess;
+    const int comps = real_in ? 1 : 2;
+    if (prec == 8)
+        hipLaunchKernelGGL(k_gather_first<double>, dim3(blocks_for(batch, 256)), dim3(256), 0, s,
+                           (const double*)in, dist_hi, dist_lo, C, (double*)out, batch, comps);
+    else
+        hipLaunchKernelGGL(k_gather_first<float>, dim3(blocks_for(batch, 256)), dim3(256), 0, s,
+                           (const float*)in, dist_hi, dist_lo, C, (float*)out, batch, comps);
+    return hipGetLastError();
+}
+
+hipError_t launch_rader_dc(int prec, const void* x0, int x0_real, const void* A, uint32_t L,
+                           void* out, int64_t dh, int64_t dl, uint32_t C, int real_out, int conj_,
+                           double scale, uint64_t batch, hipStream_t s) {
+    if (!batch)
+        return hipSuccess;
+    if (prec == 8)
+        hipLaunchKernelGGL(k_rader_dc<double>, dim3(blocks_for(batch, 256)), dim3(256), 0, s,
+                           (const double*)x0, x0_real, (const cx<double>*)A, L, (double*)out, dh,
+                           dl, C, real_out, conj_, scale, batch);
+    else
+        hipLaunchKernelGGL(k_rader_dc<float>, dim3(blocks_for(batch, 256)), dim3(256), 0, s,
+                           (const float*)x0, x0_real, (const cx<float>*)A, L, (float*)out, dh, dl,
+                           C, real_out, conj_, scale, batch);
+    return hipGetLastError();
+}
+
+} // namespace kfa

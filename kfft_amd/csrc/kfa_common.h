@@ -1,0 +1,71 @@
+// Shared host/device definitions of the gfx950 transform engine.
+//
+// One kernel family (k_fft) runs a batched length-L complex FFT entirely in a
+// workgroup: a cooperative load moves the workgroup's instances from HBM into
+// LDS, the radix stages run LDS to LDS (Stockham autosort), and a cooperative
+// store writes them back.  Planner-built pass descriptors (KfaFftArgs) describe how element e of
+// instance b is addressed on each side and which fused element operations
+// (conjugation, pointwise products, four-step twiddles, Rader gather/scatter,
+// Bluestein chirps, real in/out) apply on the way in and out.
+#pragma once
+#include <stdint.h>
+
+#define KFA_MAX_STAGES 24
+#define KFA_MAX_RADIX 32
+
+// I/O flags (KfaIo::flags)
+#define KFA_IO_REAL 0x0001u       // real data: load imag=0 / store real part
+#define KFA_IO_REAL_PAIRS 0x0002u // store complex as two reals at 2*ph, 2*ph+1 (stride in reals)
+#define KFA_IO_CONJ1 0x0004u      // conjugate right after load / right before store ops
+#define KFA_IO_CONJ2 0x0008u      // conjugate at the end of load ops / after scaling on store
+#define KFA_IO_PW 0x0010u         // pointwise multiply by pw[...]
+#define KFA_IO_PW_LAYOUT 0x0020u  // pw indexed like the data (else by logical index)
+#define KFA_IO_SCALE 0x0040u      // multiply by KfaFftArgs::scale (store)
+#define KFA_IO_TW4 0x0080u        // four-step twiddle W_N^(b_lo * e) (store)
+#define KFA_IO_ADDX0 0x0100u      // add the instance's aux value (Rader x[0]) after all ops (store)
+#define KFA_IO_MAP 0x0800u        // element permutation map[logical]
+#define KFA_IO_PW_F64 0x1000u     // pw table is double precision complex
+#define KFA_IO_SCALE_POST 0x2000u // multiply by KfaFftArgs::scale_post after ADDX0
+#define KFA_IO_AUX_REAL 0x4000u   // aux (x[0] source) is a real array
+
+// Addressing of element e of instance b on one side of a pass:
+//   b_lo = b % C, b_hi = b / C
+//   lg   = b_lo * lo_mul + e * e_mul              (logical index)
+//   ph   = (flags & MAP) ? map[lg] : lg           (physical index)
+//   addr = b_hi * dist_hi + b_lo * dist_lo + ph * stride   (in elements)
+// lg >= valid loads zero / skips the store.
+struct KfaIo {
+    uint64_t ptr;
+    int64_t dist_hi, dist_lo, stride;
+    uint64_t map;
+    uint64_t pw;
+    uint64_t aux; // ADDX0: x[0] values, indexed by b_hi*aux_dist + b_lo*aux_dist_lo
+    int64_t aux_dist, aux_dist_lo;
+    uint32_t C, lo_mul, e_mul, valid;
+    uint32_t flags, pad0;
+};
+
+struct KfaStage {
+    uint32_t radix, ns, tw_off, pad;
+};
+
+struct KfaFftArgs {
+    KfaIo in, out;
+    uint64_t tw;               // per-stage twiddles, complex of the compute precision
+    uint64_t tw4_hi, tw4_lo;   // four-step twiddle tables, complex double
+    double scale, scale_post;
+    uint64_t batch;            // number of instances
+    uint32_t tw4_n, tw4_shift; // four-step length and log2 of the low table size
+    uint32_t L, nstages;
+    uint32_t tpi, ipw;         // threads per instance, instances per workgroup
+    uint32_t in_jfast;         // cooperative load: consecutive threads on consecutive instances
+    uint32_t out_jfast;        // cooperative store: same for the store side
+    uint32_t inst_stride;      // LDS elements per instance (padded, odd)
+    uint32_t pad1;
+    KfaStage st[KFA_MAX_STAGES];
+};
+
+// LDS padding: one spare element every 2^KFA_PAD_SHIFT elements keeps the
+// stride-R writes of a Stockham stage off a single bank group.
+#define KFA_PAD_SHIFT_F64 4
+#define KFA_PAD_SHIFT_F32 5

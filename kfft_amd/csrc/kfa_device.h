@@ -1,0 +1,327 @@
+// Device-side building blocks of the transform engine: complex arithmetic,
+// radix codelets and the fused element load/store operations of a pass.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kfa_common.h"
+
+namespace kfa {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cx {
+    T re, im;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) {
+    return {a.re + b.re, a.im + b.im};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) {
+    return {a.re - b.re, a.im - b.im};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> mul_mi(cx<T> a) { // a * (-i)
+    return {a.im, -a.re};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> conj(cx<T> a) {
+    return {a.re, -a.im};
+}
+
+// ---------------------------------------------------------------------------
+// compile-time cos/sin of 2*pi*m/R: octant reduction on exact integers, then a
+// Taylor series on |phi| <= pi/4
+// ---------------------------------------------------------------------------
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+constexpr double taylor_sin(double x) {
+    double t = x, s = x;
+    for (int n = 1; n < 14; ++n) {
+        t *= -x * x / ((2.0 * n) * (2.0 * n + 1));
+        s += t;
+    }
+    return s;
+}
+constexpr double taylor_cos(double x) {
+    double t = 1, s = 1;
+    for (int n = 1; n < 14; ++n) {
+        t *= -x * x / ((2.0 * n - 1) * (2.0 * n));
+        s += t;
+    }
+    return s;
+}
+struct CS {
+    double c, s;
+};
+constexpr CS cs_turn(long m, long R) {
+    long u = (8 * (m % R) + 8 * R) % (8 * R); // angle = (pi/4) * u / R
+    long o = u / R, rem = u % R;
+    long quad = 0;
+    double phi = 0;
+    int sg = 1;
+    if (o % 2 == 0) {
+        quad = o / 2;
+        phi = (kPi / 4) * (double)rem / (double)R;
+        sg = 1;
+    } else {
+        quad = (o + 1) / 2;
+        phi = (kPi / 4) * (double)(R - rem) / (double)R;
+        sg = -1;
+    }
+    double cp = taylor_cos(phi), sp = taylor_sin(phi);
+    double ca = 0, sa = 0;
+    switch (quad % 4) {
+    case 0: ca = 1; break;
+    case 1: sa = 1; break;
+    case 2: ca = -1; break;
+    default: sa = -1; break;
+    }
+    return {ca * cp - sa * sg * sp, sa * cp + ca * sg * sp};
+}
+template <int R>
+struct OddTab {
+    double c[R], s[R];
+    constexpr OddTab() : c(), s() {
+        for (int m = 0; m < R; ++m) {
+            CS v = cs_turn(m, R);
+            c[m] = v.c;
+            s[m] = v.s;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// radix codelets: forward DFT of v[0..R-1] in natural order, in place
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void dft4(cx<T>& a0, cx<T>& a1, cx<T>& a2, cx<T>& a3) {
+    cx<T> s02 = a0 + a2, d02 = a0 - a2, s13 = a1 + a3, d13 = mul_mi(a1 - a3);
+    a0 = s02 + s13;
+    a2 = s02 - s13;
+    a1 = d02 + d13;
+    a3 = d02 - d13;
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void dft_odd(cx<T>* v) {
+    constexpr int H = (R - 1) / 2;
+    constexpr OddTab<R> tab{};
+    cx<T> s[H + 1], d[H + 1];
+    cx<T> y0 = v[0];
+#pragma unroll
+    for (int j = 1; j <= H; ++j) {
+        s[j] = v[j] + v[R - j];
+        d[j] = v[j] - v[R - j];
+        y0 = y0 + s[j];
+    }
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+        cx<T> ra = v[0], ia = {T(0), T(0)};
+#pragma unroll
+        for (int j = 1; j <= H; ++j) {
+            const int m = (j * k) % R;
+            const T c = (T)tab.c[m], sn = (T)tab.s[m];
+            ra.re += s[j].re * c;
+            ra.im += s[j].im * c;
+            ia.re += d[j].re * sn;
+            ia.im += d[j].im * sn;
+        }
+        v[k] = {ra.re + ia.im, ra.im - ia.re};
+        v[R - k] = {ra.re - ia.im, ra.im + ia.re};
+    }
+    v[0] = y0;
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void dft(cx<T>* v) {
+    if constexpr (R == 2) {
+        cx<T> a = v[0], b = v[1];
+        v[0] = a + b;
+        v[1] = a - b;
+    } else if constexpr (R == 4) {
+        dft4(v[0], v[1], v[2], v[3]);
+    } else if constexpr (R == 8) {
+        const T r = (T)0.707106781186547524400844362104849039;
+        cx<T> a0 = v[0], a1 = v[2], a2 = v[4], a3 = v[6];
+        cx<T> b0 = v[1], b1 = v[3], b2 = v[5], b3 = v[7];
+        dft4(a0, a1, a2, a3);
+        dft4(b0, b1, b2, b3);
+        b1 = {(b1.re + b1.im) * r, (b1.im - b1.re) * r};
+        b2 = mul_mi(b2);
+        b3 = {(b3.im - b3.re) * r, -(b3.re + b3.im) * r};
+        v[0] = a0 + b0;
+        v[4] = a0 - b0;
+        v[1] = a1 + b1;
+        v[5] = a1 - b1;
+        v[2] = a2 + b2;
+        v[6] = a2 - b2;
+        v[3] = a3 + b3;
+        v[7] = a3 - b3;
+    } else if constexpr (R == 16) {
+        const T c1 = (T)0.923879532511286756128183189396788933;
+        const T s1 = (T)0.382683432365089771728459984030398866;
+        const T r2 = (T)0.707106781186547524400844362104849039;
+        cx<T> a[4][4]; // a[n2][k1]
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) {
+            a[n2][0] = v[n2];
+            a[n2][1] = v[4 + n2];
+            a[n2][2] = v[8 + n2];
+            a[n2][3] = v[12 + n2];
+            dft4(a[n2][0], a[n2][1], a[n2][2], a[n2][3]);
+        }
+        // W16^(n2*k1)
+        a[1][1] = cmul(a[1][1], cx<T>{c1, -s1});
+        a[1][2] = cmul(a[1][2], cx<T>{r2, -r2});
+        a[1][3] = cmul(a[1][3], cx<T>{s1, -c1});
+        a[2][1] = cmul(a[2][1], cx<T>{r2, -r2});
+        a[2][2] = mul_mi(a[2][2]);
+        a[2][3] = cmul(a[2][3], cx<T>{-r2, -r2});
+        a[3][1] = cmul(a[3][1], cx<T>{s1, -c1});
+        a[3][2] = cmul(a[3][2], cx<T>{-r2, -r2});
+        a[3][3] = cmul(a[3][3], cx<T>{-c1, s1});
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            cx<T> u0 = a[0][k1], u1 = a[1][k1], u2 = a[2][k1], u3 = a[3][k1];
+            dft4(u0, u1, u2, u3);
+            v[k1] = u0;
+            v[k1 + 4] = u1;
+            v[k1 + 8] = u2;
+            v[k1 + 12] = u3;
+        }
+    } else {
+        static_assert(R % 2 == 1, "unsupported even radix");
+        dft_odd<T, R>(v);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fused load / store
+// ---------------------------------------------------------------------------
+struct Ctx {
+    int64_t base;  // b_hi*dist_hi + b_lo*dist_lo
+    uint32_t lo;   // b_lo
+    uint32_t lgb;  // b_lo * lo_mul
+    uint64_t hi;   // b_hi
+};
+
+__device__ __forceinline__ Ctx make_ctx(const KfaIo& io, uint64_t inst) {
+    Ctx c;
+    uint64_t hi;
+    uint32_t lo;
+    if (io.C == 1) {
+        hi = inst;
+        lo = 0;
+    } else if (inst <= 0xffffffffull) {
+        const uint32_t i32 = (uint32_t)inst, h32 = i32 / io.C;
+        hi = h32;
+        lo = i32 - h32 * io.C;
+    } else {
+        hi = inst / io.C;
+        lo = (uint32_t)(inst - hi * io.C);
+    }
+    c.base = (int64_t)hi * io.dist_hi + (int64_t)lo * io.dist_lo;
+    c.lo = lo;
+    c.lgb = lo * io.lo_mul;
+    c.hi = hi;
+    return c;
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> pw_value(const KfaIo& io, uint32_t lg, int64_t addr) {
+    const uint64_t idx = (io.flags & KFA_IO_PW_LAYOUT) ? (uint64_t)addr : (uint64_t)lg;
+    if (io.flags & KFA_IO_PW_F64) {
+        const cx<double> w = ((const cx<double>*)io.pw)[idx];
+        return {(T)w.re, (T)w.im};
+    }
+    return ((const cx<T>*)io.pw)[idx];
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> load_aux(const KfaIo& io, const Ctx& c) {
+    const int64_t i = (int64_t)c.hi * io.aux_dist + (int64_t)c.lo * io.aux_dist_lo;
+    if (io.flags & KFA_IO_AUX_REAL)
+        return {((const T*)io.aux)[i], T(0)};
+    return ((const cx<T>*)io.aux)[i];
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> load_elem(const KfaIo& io, const Ctx& c, uint32_t e) {
+    const uint32_t lg = c.lgb + e * io.e_mul;
+    if (lg >= io.valid)
+        return {T(0), T(0)};
+    const uint32_t ph = (io.flags & KFA_IO_MAP) ? ((const uint32_t*)io.map)[lg] : lg;
+    const int64_t addr = c.base + (int64_t)ph * io.stride;
+    cx<T> v;
+    if (io.flags & KFA_IO_REAL) {
+        v.re = ((const T*)io.ptr)[addr];
+        v.im = T(0);
+    } else {
+        v = ((const cx<T>*)io.ptr)[addr];
+    }
+    if (io.flags & KFA_IO_CONJ1)
+        v.im = -v.im;
+    if (io.flags & KFA_IO_PW)
+        v = cmul(v, pw_value<T>(io, lg, addr));
+    if (io.flags & KFA_IO_CONJ2)
+        v.im = -v.im;
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ void store_elem(const KfaFftArgs& A, const Ctx& c, uint32_t e, cx<T> v,
+                                           cx<T> x0) {
+    const KfaIo& io = A.out;
+    const uint32_t lg = c.lgb + e * io.e_mul;
+    if (lg >= io.valid)
+        return;
+    if (io.flags & KFA_IO_CONJ1)
+        v.im = -v.im;
+    if (io.flags & KFA_IO_TW4) {
+        // W_N^(b_lo * e) from two double tables; one rounding into T
+        const uint32_t m = c.lo * e;
+        const cx<double> h = ((const cx<double>*)A.tw4_hi)[m >> A.tw4_shift];
+        const cx<double> l = ((const cx<double>*)A.tw4_lo)[m & ((1u << A.tw4_shift) - 1u)];
+        const cx<double> w = cmul(h, l);
+        v = {(T)((double)v.re * w.re - (double)v.im * w.im),
+             (T)((double)v.re * w.im + (double)v.im * w.re)};
+    }
+    const uint32_t ph = (io.flags & KFA_IO_MAP) ? ((const uint32_t*)io.map)[lg] : lg;
+    const int64_t addr = c.base + (int64_t)ph * io.stride;
+    if (io.flags & KFA_IO_PW)
+        v = cmul(v, pw_value<T>(io, lg, addr));
+    if (io.flags & KFA_IO_SCALE) {
+        v.re *= (T)A.scale;
+        v.im *= (T)A.scale;
+    }
+    if (io.flags & KFA_IO_CONJ2)
+        v.im = -v.im;
+    if (io.flags & KFA_IO_ADDX0)
+        v = v + x0;
+    if (io.flags & KFA_IO_SCALE_POST) {
+        v.re *= (T)A.scale_post;
+        v.im *= (T)A.scale_post;
+    }
+    if (io.flags & KFA_IO_REAL) {
+        ((T*)io.ptr)[addr] = v.re;
+    } else if (io.flags & KFA_IO_REAL_PAIRS) {
+        T* o = (T*)io.ptr;
+        o[c.base + (int64_t)(2 * ph) * io.stride] = v.re;
+        o[c.base + (int64_t)(2 * ph + 1) * io.stride] = v.im;
+    } else {
+        ((cx<T>*)io.ptr)[addr] = v;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ uint32_t lds_index(uint32_t e) {
+    return e + (e >> (sizeof(T) == 8 ? KFA_PAD_SHIFT_F64 : KFA_PAD_SHIFT_F32));
+}
+
+
+} // namespace kfa

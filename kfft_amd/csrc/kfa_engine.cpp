@@ -1,0 +1,688 @@
+// Planner and executor of the gfx950 transform engine (see kfa_engine.h).
+#include "kfa_engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+
+#include "kfa_kernels.h"
+
+namespace kfa {
+
+// ===================================================================== runtime
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+const char* last_error() { return g_err.c_str(); }
+
+bool hip_ok(hipError_t e, const char* what) {
+    if (e == hipSuccess)
+        return true;
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return false;
+}
+
+static std::mutex g_rt_mu;
+static hipStream_t g_stream = nullptr;
+static bool g_rt_ready = false;
+
+bool runtime_init() {
+    std::lock_guard<std::mutex> lk(g_rt_mu);
+    if (g_rt_ready)
+        return true;
+    int n = 0;
+    if (!hip_ok(hipGetDeviceCount(&n), "hipGetDeviceCount"))
+        return false;
+    if (n <= 0) {
+        set_error("no HIP device visible");
+        return false;
+    }
+    if (!hip_ok(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking), "hipStreamCreate"))
+        return false;
+    g_rt_ready = true;
+    return true;
+}
+
+hipStream_t default_stream() {
+    runtime_init();
+    return g_stream;
+}
+
+DevBuf::~DevBuf() { release(); }
+void DevBuf::release() {
+    if (p) {
+        // work queued on the library stream may still read this buffer
+        if (g_stream)
+            (void)hipStreamSynchronize(g_stream);
+        (void)hipFree(p);
+    }
+    p = nullptr;
+    bytes = 0;
+}
+bool DevBuf::alloc(size_t n) {
+    release();
+    if (n == 0)
+        return true;
+    if (!hip_ok(hipMalloc(&p, n), "hipMalloc")) {
+        p = nullptr;
+        return false;
+    }
+    bytes = n;
+    return true;
+}
+bool DevBuf::ensure(size_t n) {
+    if (n <= bytes)
+        return true;
+    return alloc(n);
+}
+
+size_t elem_bytes(int prec) { return 2 * (size_t)prec; }
+
+// ================================================================== LDS passes
+static uint32_t pad_index(uint32_t e, int prec) {
+    return e + (e >> (prec == 8 ? KFA_PAD_SHIFT_F64 : KFA_PAD_SHIFT_F32));
+}
+
+std::vector<uint32_t> radix_list(uint32_t L) {
+    std::vector<uint32_t> r;
+    if (L <= 1)
+        return r;
+    uint32_t a = 0, b = 0, m = L;
+    while (m % 2 == 0) {
+        m /= 2;
+        ++a;
+    }
+    while (m % 3 == 0) {
+        m /= 3;
+        ++b;
+    }
+    std::vector<uint32_t> odd;
+    for (uint32_t p : {5u, 7u, 11u, 13u, 17u, 19u, 23u, 29u, 31u})
+        while (m % p == 0) {
+            m /= p;
+            odd.push_back(p);
+        }
+    if (m != 1)
+        return {};
+    const uint32_t q = a / 4, rr = a % 4;
+    if (rr == 1 && q > 0) {
+        for (uint32_t i = 0; i + 1 < q; ++i)
+            r.push_back(16);
+        r.push_back(8);
+        r.push_back(4);
+    } else {
+        for (uint32_t i = 0; i < q; ++i)
+            r.push_back(16);
+        if (rr == 1)
+            r.push_back(2);
+        else if (rr == 2)
+            r.push_back(4);
+        else if (rr == 3)
+            r.push_back(8);
+    }
+    for (uint32_t i = 0; i < b / 2; ++i)
+        r.push_back(9);
+    if (b % 2)
+        r.push_back(3);
+    for (uint32_t p : odd)
+        r.push_back(p);
+    return r;
+}
+
+static bool lds_shape(uint32_t L, int prec, int* maxe_out, uint32_t* tpi_out, uint32_t* stride_out) {
+    if (L == 1) {
+        *maxe_out = 16;
+        *tpi_out = 1;
+        *stride_out = 1;
+        return true;
+    }
+    std::vector<uint32_t> rl = radix_list(L);
+    if (rl.empty() || rl.size() > KFA_MAX_STAGES)
+        return false;
+    const uint32_t rmax = *std::max_element(rl.begin(), rl.end());
+    const int maxe = rmax > 16 ? 32 : 16;
+    uint32_t tpi = 1;
+    for (uint32_t R : rl) {
+        const uint32_t per = std::max<uint32_t>(1, (uint32_t)maxe / R);
+        const uint32_t need = (L / R + per - 1) / per;
+        tpi = std::max(tpi, need);
+    }
+    const uint32_t limit = maxe == 16 ? 512 : 256;
+    if (tpi > limit)
+        return false;
+    uint32_t stride = pad_index(L - 1, prec) + 1;
+    stride |= 1u;
+    if ((size_t)stride * elem_bytes(prec) > 160 * 1024)
+        return false;
+    *maxe_out = maxe;
+    *tpi_out = tpi;
+    *stride_out = stride;
+    return true;
+}
+
+bool lds_capable(uint32_t L, int prec) {
+    int m;
+    uint32_t t, s;
+    return lds_shape(L, prec, &m, &t, &s);
+}
+
+bool LdsPass::plan(uint32_t len, int p) {
+    L = len;
+    prec = p;
+    if (!lds_shape(L, prec, &maxe, &tpi, &inst_stride)) {
+        set_error("length " + std::to_string(L) + " does not fit one LDS pass");
+        return false;
+    }
+    std::vector<uint32_t> rl = radix_list(L);
+    nstages = (uint32_t)rl.size();
+    uint32_t ns = 1, off = 0;
+    for (uint32_t s = 0; s < nstages; ++s) {
+        st[s].radix = rl[s];
+        st[s].ns = ns;
+        st[s].tw_off = off;
+        if (s > 0)
+            off += ns * (rl[s] - 1);
+        ns *= rl[s];
+    }
+    if (nstages >= 2) {
+        if (!tw.alloc((size_t)std::max<uint32_t>(off, 1) * elem_bytes(prec)))
+            return false;
+        KfaFftArgs a;
+        memset(&a, 0, sizeof(a));
+        a.L = L;
+        a.nstages = nstages;
+        memcpy(a.st, st, sizeof(st));
+        if (!hip_ok(launch_stage_twiddles(prec, tw.p, a, default_stream()), "stage twiddles"))
+            return false;
+    }
+    return true;
+}
+
+uint32_t LdsPass::lds_bytes(uint32_t ipw) const {
+    if (nstages == 0 && L <= 1)
+        return (uint32_t)(ipw * inst_stride * elem_bytes(prec));
+    return (uint32_t)(ipw * inst_stride * elem_bytes(prec));
+}
+
+uint32_t LdsPass::pick_ipw(bool jfast) const {
+    const uint32_t limit = maxe == 16 ? 512 : 256;
+    uint32_t ipw = std::max<uint32_t>(1, 256 / tpi);
+    if (jfast)
+        ipw = std::max<uint32_t>(ipw, 16);
+    while (ipw > 1 && tpi * ipw > limit)
+        --ipw;
+    const size_t per = (size_t)inst_stride * elem_bytes(prec);
+    const size_t budget = 72 * 1024;
+    while (ipw > 1 && per * ipw > budget)
+        --ipw;
+    return ipw;
+}
+
+// ==================================================================== SubFft
+static bool split_four(uint32_t L, int prec, uint32_t* n1, uint32_t* n2) {
+    uint32_t best = 0;
+    for (uint32_t d = (uint32_t)std::sqrt((double)L) + 1; d >= 2; --d) {
+        if (L % d)
+            continue;
+        const uint32_t a = std::min(d, L / d), b = std::max(d, L / d);
+        if (lds_capable(a, prec) && lds_capable(b, prec)) {
+            best = a;
+            break;
+        }
+    }
+    if (!best)
+        return false;
+    *n1 = best;
+    *n2 = L / best;
+    return true;
+}
+
+static uint32_t direct_max(int prec) { return prec == 8 ? 4096u : 8192u; }
+
+bool subfft_capable(uint32_t L, int prec) {
+    if (lds_capable(L, prec))
+        return true;
+    uint32_t a, b;
+    return split_four(L, prec, &a, &b);
+}
+
+bool SubFft::plan(uint32_t len, int p, bool force_four) {
+    L = len;
+    prec = p;
+    if (!force_four && lds_capable(L, prec) && (L <= direct_max(prec) || !split_four(L, prec, &N1, &N2))) {
+        four = false;
+        return direct.plan(L, prec);
+    }
+    if (!split_four(L, prec, &N1, &N2)) {
+        set_error("length " + std::to_string(L) + " has no two-pass split");
+        return false;
+    }
+    four = true;
+    if (!col.plan(N1, prec) || !row.plan(N2, prec))
+        return false;
+    // two-level table for W_L^m: m = hi*S + lo, S = 2^shift >= sqrt(L)
+    uint32_t sh = 0;
+    while ((1ull << (2 * sh)) < L)
+        ++sh;
+    tw4_shift = sh;
+    const uint64_t S = 1ull << sh, nhi = (L + S - 1) / S + 1;
+    if (!tw4_lo.alloc(S * 16) || !tw4_hi.alloc(nhi * 16))
+        return false;
+    hipStream_t s = default_stream();
+    if (!hip_ok(launch_twiddle_table(tw4_lo.p, S, 1, L, s), "tw4 lo") ||
+        !hip_ok(launch_twiddle_table(tw4_hi.p, nhi, S, L, s), "tw4 hi"))
+        return false;
+    return true;
+}
+
+static bool io_jfast(const KfaIo& io) {
+    if (io.stride == 1 || io.stride == -1)
+        return false;
+    if (io.C > 1)
+        return io.dist_lo == 1 || io.dist_lo == -1;
+    return io.dist_hi == 1 || io.dist_hi == -1;
+}
+
+static hipError_t run_pass(const LdsPass& P, const KfaIo& in, const KfaIo& out, double scale,
+                           double scale_post, uint64_t batch, const void* tw4_hi,
+                           const void* tw4_lo, uint32_t tw4_n, uint32_t tw4_shift, hipStream_t s) {
+    KfaFftArgs a;
+    memset(&a, 0, sizeof(a));
+    a.in = in;
+    a.out = out;
+    a.tw = (uint64_t)P.tw.p;
+    a.tw4_hi = (uint64_t)tw4_hi;
+    a.tw4_lo = (uint64_t)tw4_lo;
+    a.tw4_n = tw4_n;
+    a.tw4_shift = tw4_shift;
+    a.scale = scale;
+    a.scale_post = scale_post;
+    a.batch = batch;
+    a.L = P.L;
+    a.nstages = P.nstages;
+    a.tpi = P.tpi;
+    a.in_jfast = io_jfast(in);
+    a.out_jfast = io_jfast(out);
+    a.ipw = P.pick_ipw(a.in_jfast || a.out_jfast);
+    a.inst_stride = P.inst_stride;
+    memcpy(a.st, P.st, sizeof(a.st));
+    return launch_fft(P.prec, P.maxe, a, P.lds_bytes(a.ipw), s);
+}
+
+hipError_t SubFft::run(const KfaIo& load, const KfaIo& store, double scale, double scale_post,
+                       uint64_t batch, void* scratch, hipStream_t s) const {
+    if (!four)
+        return run_pass(direct, load, store, scale, scale_post, batch, nullptr, nullptr, 0, 0, s);
+    if (load.C != 1 || store.C != 1)
+        return hipErrorInvalidValue;
+    // pass A: columns of length N1 (stride N2), twiddle W_L^(n2 k1) on store
+    KfaIo ia = load;
+    ia.C = N2;
+    ia.dist_lo = 0;
+    ia.lo_mul = 1;
+    ia.e_mul = N2;
+    ia.stride = load.stride;
+    KfaIo oa;
+    memset(&oa, 0, sizeof(oa));
+    oa.ptr = (uint64_t)scratch;
+    oa.C = N2;
+    oa.dist_hi = L;
+    oa.dist_lo = 1;
+    oa.stride = N2;
+    oa.lo_mul = 0;
+    oa.e_mul = 1;
+    oa.valid = 0xffffffffu;
+    oa.flags = KFA_IO_TW4;
+    hipError_t e = run_pass(col, ia, oa, 1.0, scale_post, batch * N2, tw4_hi.p, tw4_lo.p, L,
+                            tw4_shift, s);
+    if (e != hipSuccess)
+        return e;
+    // pass B: rows of length N2, transposed store X[k1 + N1 k2]
+    KfaIo ib;
+    memset(&ib, 0, sizeof(ib));
+    ib.ptr = (uint64_t)scratch;
+    ib.C = N1;
+    ib.dist_hi = L;
+    ib.dist_lo = N2;
+    ib.stride = 1;
+    ib.lo_mul = 0;
+    ib.e_mul = 1;
+    ib.valid = 0xffffffffu;
+    KfaIo ob = store;
+    ob.C = N1;
+    ob.dist_lo = 0;
+    ob.aux_dist_lo = 0;
+    ob.lo_mul = 1;
+    ob.e_mul = N1;
+    return run_pass(row, ib, ob, scale, scale_post, batch * N1, nullptr, nullptr, 0, 0, s);
+}
+
+// ==================================================================== Plan1D
+static bool is_prime(uint32_t n) {
+    if (n < 2)
+        return false;
+    for (uint64_t d = 2; d * d <= n; ++d)
+        if (n % d == 0)
+            return false;
+    return true;
+}
+
+static uint64_t powmod(uint64_t b, uint64_t e, uint64_t m) {
+    uint64_t r = 1 % m;
+    b %= m;
+    while (e) {
+        if (e & 1)
+            r = r * b % m;
+        b = b * b % m;
+        e >>= 1;
+    }
+    return r;
+}
+
+static uint32_t primitive_root(uint32_t p) {
+    std::vector<uint32_t> f;
+    uint32_t m = p - 1;
+    for (uint32_t d = 2; (uint64_t)d * d <= m; ++d)
+        if (m % d == 0) {
+            f.push_back(d);
+            while (m % d == 0)
+                m /= d;
+        }
+    if (m > 1)
+        f.push_back(m);
+    for (uint32_t g = 2; g < p; ++g) {
+        bool ok = true;
+        for (uint32_t q : f)
+            if (powmod(g, (p - 1) / q, p) == 1) {
+                ok = false;
+                break;
+            }
+        if (ok)
+            return g;
+    }
+    return 0;
+}
+
+static KfaIo io_plain(void* ptr, int64_t dist_hi) {
+    KfaIo io;
+    memset(&io, 0, sizeof(io));
+    io.ptr = (uint64_t)ptr;
+    io.C = 1;
+    io.dist_hi = dist_hi;
+    io.stride = 1;
+    io.e_mul = 1;
+    io.valid = 0xffffffffu;
+    return io;
+}
+
+static KfaIo io_from(const Layout& l) {
+    KfaIo io;
+    memset(&io, 0, sizeof(io));
+    io.ptr = (uint64_t)l.ptr;
+    io.C = l.C ? l.C : 1;
+    io.dist_hi = l.dist_hi;
+    io.dist_lo = l.dist_lo;
+    io.stride = l.stride;
+    io.e_mul = 1;
+    io.valid = 0xffffffffu;
+    io.flags = l.flags & (KFA_IO_REAL | KFA_IO_REAL_PAIRS);
+    return io;
+}
+
+// FFT of a double-precision table with a temporary double-precision plan,
+// then rounded once into the plan precision.
+static bool fft_table_f64(uint32_t L, const void* src64, DevBuf& dst, int prec, double scale) {
+    SubFft f;
+    if (!f.plan(L, 8))
+        return false;
+    DevBuf out64, scr;
+    if (!out64.alloc((size_t)L * 16) || !scr.alloc(std::max<size_t>(16, f.scratch_elems(1) * 16)))
+        return false;
+    hipStream_t s = default_stream();
+    KfaIo in = io_plain((void*)src64, L), out = io_plain(out64.p, L);
+    if (!hip_ok(f.run(in, out, 1.0, 1.0, 1, scr.p, s), "table fft"))
+        return false;
+    if (!dst.alloc((size_t)L * elem_bytes(prec)))
+        return false;
+    if (!hip_ok(launch_convert_from_f64(prec, dst.p, out64.p, L, scale, s), "convert"))
+        return false;
+    return hip_ok(hipStreamSynchronize(s), "table sync");
+}
+
+static uint32_t smooth_at_least(uint64_t n, int prec) {
+    for (uint64_t m = n; m < (1ull << 32); ++m) {
+        uint64_t r = m;
+        for (uint64_t p : {2ull, 3ull, 5ull})
+            while (r % p == 0)
+                r /= p;
+        if (r == 1 && subfft_capable((uint32_t)m, prec))
+            return (uint32_t)m;
+    }
+    return 0;
+}
+
+bool Plan1D::build() {
+    if (!runtime_init())
+        return false;
+    if (n_ == 0) {
+        set_error("zero length");
+        return false;
+    }
+    hipStream_t s = default_stream();
+    if (n_ == 1 || subfft_capable(n_, prec_)) {
+        kind_ = n_ == 1 ? Kind::Identity : Kind::Direct;
+        if (n_ == 1) {
+            sub_.L = 1;
+            sub_.prec = prec_;
+            sub_.four = false;
+            return sub_.direct.plan(1, prec_);
+        }
+        return sub_.plan(n_, prec_);
+    }
+    if (is_prime(n_) && subfft_capable(n_ - 1, prec_)) {
+        kind_ = Kind::Rader;
+        const uint32_t L = n_ - 1, g = primitive_root(n_);
+        const uint32_t ginv = (uint32_t)powmod(g, n_ - 2, n_);
+        std::vector<uint32_t> q(L), p(L);
+        uint64_t a = 1, b = 1;
+        for (uint32_t k = 0; k < L; ++k) {
+            q[k] = (uint32_t)a;
+            p[k] = (uint32_t)b;
+            a = a * g % n_;
+            b = b * ginv % n_;
+        }
+        if (!qidx_.alloc(L * 4) || !pidx_.alloc(L * 4))
+            return false;
+        if (!hip_ok(hipMemcpy(qidx_.p, q.data(), L * 4, hipMemcpyHostToDevice), "qidx") ||
+            !hip_ok(hipMemcpy(pidx_.p, p.data(), L * 4, hipMemcpyHostToDevice), "pidx"))
+            return false;
+        DevBuf b64;
+        if (!b64.alloc((size_t)L * 16))
+            return false;
+        if (!hip_ok(launch_twiddle_indexed(b64.p, (const uint32_t*)pidx_.p, L, n_, 0, s), "rader b"))
+            return false;
+        if (!fft_table_f64(L, b64.p, bhat_, prec_, 1.0))
+            return false;
+        return sub_.plan(L, prec_);
+    }
+    kind_ = Kind::Bluestein;
+    M_ = smooth_at_least(2ull * n_ - 1, prec_);
+    if (!M_) {
+        set_error("no Bluestein length for " + std::to_string(n_));
+        return false;
+    }
+    DevBuf c64, h64;
+    if (!c64.alloc((size_t)n_ * 16) || !h64.alloc((size_t)M_ * 16))
+        return false;
+    if (!hip_ok(launch_bluestein_tables(c64.p, h64.p, n_, M_, s), "bluestein tables"))
+        return false;
+    if (!fft_table_f64(M_, h64.p, hhat_, prec_, 1.0))
+        return false;
+    if (!chirp_.alloc((size_t)n_ * elem_bytes(prec_)))
+        return false;
+    if (!hip_ok(launch_convert_from_f64(prec_, chirp_.p, c64.p, n_, 1.0, s), "chirp"))
+        return false;
+    if (!hip_ok(hipStreamSynchronize(s), "bluestein sync"))
+        return false;
+    return sub_.plan(M_, prec_);
+}
+
+const char* Plan1D::describe() const {
+    char buf[256];
+    const char* k = kind_ == Kind::Identity ? "identity"
+                    : kind_ == Kind::Direct ? "direct"
+                    : kind_ == Kind::Rader  ? "rader"
+                                            : "bluestein";
+    if (sub_.four)
+        snprintf(buf, sizeof buf, "%s n=%u inner=%u four-step %ux%u", k, n_, sub_.L, sub_.N1,
+                 sub_.N2);
+    else
+        snprintf(buf, sizeof buf, "%s n=%u inner=%u one-pass", k, n_, sub_.L);
+    desc_ = buf;
+    return desc_.c_str();
+}
+
+bool Plan1D::supports_columns() const { return !sub_.four; }
+
+size_t Plan1D::scratch_bytes(uint64_t batch) const {
+    const size_t eb = elem_bytes(prec_);
+    size_t elems = sub_.scratch_elems(batch);
+    if (kind_ == Kind::Rader)
+        elems += batch + (size_t)batch * sub_.L;
+    else if (kind_ == Kind::Bluestein)
+        elems += (size_t)batch * sub_.L;
+    return elems * eb + 256;
+}
+
+hipError_t Plan1D::execute(const Layout& in, const Layout& out, uint64_t batch, bool inverse,
+                           double scale, void* scratch, hipStream_t s, const Fuse* fuse) const {
+    if (batch == 0)
+        return hipSuccess;
+    const size_t eb = elem_bytes(prec_);
+    KfaIo load = io_from(in), store = io_from(out);
+    const bool scaled = scale != 1.0;
+    if (kind_ == Kind::Identity || kind_ == Kind::Direct) {
+        if (fuse && fuse->pw_in) {
+            load.pw = (uint64_t)fuse->pw_in;
+            load.flags |= KFA_IO_PW | KFA_IO_PW_LAYOUT | (fuse->pw_in_f64 ? KFA_IO_PW_F64 : 0);
+            if (inverse)
+                load.flags |= KFA_IO_CONJ2;
+        } else if (inverse) {
+            load.flags |= KFA_IO_CONJ1;
+        }
+        if (inverse)
+            store.flags |= KFA_IO_CONJ2;
+        if (scaled)
+            store.flags |= KFA_IO_SCALE;
+        return sub_.run(load, store, scale, 1.0, batch, scratch, s);
+    }
+    if (fuse && fuse->pw_in)
+        return hipErrorInvalidValue; // callers apply the product separately for these kinds
+    const uint32_t L = sub_.L;
+    char* sp = (char*)scratch;
+    if (kind_ == Kind::Rader) {
+        const bool real_in = (in.flags & KFA_IO_REAL) != 0;
+        const bool real_out = (out.flags & KFA_IO_REAL) != 0;
+        void* x0 = sp;
+        sp += ((batch * eb + 255) / 256) * 256;
+        void* tmpA = sp;
+        sp += (size_t)batch * L * eb;
+        void* sub_scr = sp;
+        hipError_t e = launch_gather_first(prec_, real_in, in.ptr, in.dist_hi, in.dist_lo,
+                                           in.C ? in.C : 1, x0, batch, s);
+        if (e != hipSuccess)
+            return e;
+        // pass 1: a[n] = x'[g^n], A = FFT_{p-1}(a)
+        KfaIo l1 = load;
+        l1.flags |= KFA_IO_MAP | (inverse ? KFA_IO_CONJ1 : 0);
+        l1.map = (uint64_t)qidx_.p;
+        KfaIo s1 = io_plain(tmpA, L);
+        e = sub_.run(l1, s1, 1.0, 1.0, batch, sub_scr, s);
+        if (e != hipSuccess)
+            return e;
+        // pass 2: c = IFFT(A * FFT(b)); X[g^-m] = x0 + c[m]
+        KfaIo l2 = io_plain(tmpA, L);
+        l2.flags |= KFA_IO_PW | KFA_IO_CONJ2;
+        l2.pw = (uint64_t)bhat_.p;
+        KfaIo s2 = store;
+        s2.flags |= KFA_IO_MAP | KFA_IO_CONJ1 | KFA_IO_SCALE | KFA_IO_ADDX0 |
+                    (inverse ? KFA_IO_CONJ2 : 0) | (scaled ? KFA_IO_SCALE_POST : 0) |
+                    (real_in ? KFA_IO_AUX_REAL : 0);
+        s2.map = (uint64_t)pidx_.p;
+        s2.aux = (uint64_t)x0;
+        s2.aux_dist = (s2.C > 1) ? s2.C : 1; // x0 is indexed by instance = hi*C + lo
+        s2.aux_dist_lo = (s2.C > 1) ? 1 : 0;
+        e = sub_.run(l2, s2, 1.0 / (double)L, scale, batch, sub_scr, s);
+        if (e != hipSuccess)
+            return e;
+        // X[0] = x0 + A[0]
+        return launch_rader_dc(prec_, x0, real_in, tmpA, L, out.ptr, out.dist_hi, out.dist_lo,
+                               out.C ? out.C : 1, real_out, inverse, scale, batch, s);
+    }
+    // Bluestein
+    void* tmp = sp;
+    sp += (size_t)batch * L * eb;
+    void* sub_scr = sp;
+    KfaIo l1 = load;
+    l1.flags |= KFA_IO_PW | (inverse ? KFA_IO_CONJ1 : 0);
+    l1.pw = (uint64_t)chirp_.p;
+    l1.valid = n_;
+    KfaIo s1 = io_plain(tmp, L);
+    hipError_t e = sub_.run(l1, s1, 1.0, 1.0, batch, sub_scr, s);
+    if (e != hipSuccess)
+        return e;
+    KfaIo l2 = io_plain(tmp, L);
+    l2.flags |= KFA_IO_PW | KFA_IO_CONJ2;
+    l2.pw = (uint64_t)hhat_.p;
+    KfaIo s2 = store;
+    s2.flags |= KFA_IO_CONJ1 | KFA_IO_PW | KFA_IO_SCALE | (inverse ? KFA_IO_CONJ2 : 0);
+    s2.pw = (uint64_t)chirp_.p;
+    s2.valid = n_;
+    return sub_.run(l2, s2, scale / (double)L, 1.0, batch, sub_scr, s);
+}
+
+// =============================================================== plan registry
+static std::mutex g_reg_mu;
+static std::map<std::pair<uint32_t, int>, std::weak_ptr<Plan1D>> g_reg;
+
+std::shared_ptr<Plan1D> get_plan(uint32_t n, int prec) {
+    std::lock_guard<std::mutex> lk(g_reg_mu);
+    auto key = std::make_pair(n, prec);
+    auto it = g_reg.find(key);
+    if (it != g_reg.end()) {
+        if (auto sp = it->second.lock())
+            return sp;
+    }
+    auto p = std::make_shared<Plan1D>(n, prec);
+    if (!p->build())
+        return nullptr;
+    g_reg[key] = p;
+    return p;
+}
+
+bool make_real_twiddles(DevBuf& out, uint32_t h, int prec, bool inverse) {
+    // st[i] = exp(-+ i pi ((i+1)/h + 1/2)) = exp(-+ 2 pi i (2(i+1) + h) / (4h))
+    DevBuf t64;
+    if (!t64.alloc((size_t)h * 16) || !out.alloc((size_t)h * elem_bytes(prec)))
+        return false;
+    std::vector<uint32_t> idx(h);
+    for (uint32_t i = 0; i < h; ++i)
+        idx[i] = (uint32_t)(((uint64_t)2 * (i + 1) + h) % (4ull * h));
+    DevBuf d_idx;
+    if (!d_idx.alloc((size_t)h * 4))
+        return false;
+    hipStream_t s = default_stream();
+    if (!hip_ok(hipMemcpy(d_idx.p, idx.data(), (size_t)h * 4, hipMemcpyHostToDevice), "st idx"))
+        return false;
+    if (!hip_ok(launch_twiddle_indexed(t64.p, (const uint32_t*)d_idx.p, h, 4ull * h, inverse ? 1 : 0,
+                                       s),
+                "st"))
+        return false;
+    if (!hip_ok(launch_convert_from_f64(prec, out.p, t64.p, h, 1.0, s), "st convert"))
+        return false;
+    return hip_ok(hipStreamSynchronize(s), "st sync");
+}
+
+} // namespace kfa

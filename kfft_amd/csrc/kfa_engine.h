@@ -1,0 +1,147 @@
+// Host side of the gfx950 transform engine: plans, device tables, execution.
+//
+// A Plan1D computes batched 1D complex DFTs of one length in one precision.
+// Its shape is chosen at plan time:
+//   Direct    one LDS pass (length fits a workgroup's LDS, all radices <= 31)
+//   FourStep  N = N1*N2: column pass of length N1 with the W_N^(n2 k1) twiddle
+//             fused into its store, then a row pass of length N2 whose store
+//             writes the transposed (natural-order) result
+//   Rader     prime N: gather x[g^n] fused into a forward FFT of N-1, then an
+//             FFT of N-1 whose load multiplies by the precomputed FFT(b) and
+//             whose store scatters to X[g^-m] adding x[0]
+//   Bluestein anything else: chirp-multiplied, zero-padded FFT of a smooth
+//             M >= 2N-1, pointwise product with FFT(chirp kernel), FFT back,
+//             chirp-multiplied truncated store
+// Direction is chosen per call; the inverse is conj(F(conj(x))), applied by the
+// first pass's load and the last pass's store.  Twiddles, chirps and Rader
+// kernels are generated on the GPU in double precision and rounded once.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "kfa_common.h"
+
+namespace kfa {
+
+// --------------------------------------------------------------------- runtime
+void set_error(const std::string& msg);
+const char* last_error();
+bool hip_ok(hipError_t e, const char* what);
+hipStream_t default_stream();
+bool runtime_init();
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf();
+    bool alloc(size_t n);  // exact size, contents undefined
+    bool ensure(size_t n); // grow-only
+    void release();
+};
+
+// ------------------------------------------------------------------ addressing
+// How instance b and element e of a transform's input or output are addressed
+// (see KfaIo): b_lo = b % C, b_hi = b / C,
+//   address = b_hi*dist_hi + b_lo*dist_lo + e*stride.
+struct Layout {
+    void* ptr = nullptr;
+    int64_t dist_hi = 0, dist_lo = 0, stride = 1;
+    uint32_t C = 1;
+    uint32_t flags = 0; // KFA_IO_REAL / KFA_IO_REAL_PAIRS
+};
+
+// Extra operations a caller fuses into a transform (used by the real-FFT,
+// convolution and 2D layers).
+struct Fuse {
+    const void* pw_in = nullptr; // pointwise multiply on load, same layout as the input
+    bool pw_in_f64 = false;
+};
+
+// -------------------------------------------------------------- LDS pass plans
+struct LdsPass {
+    uint32_t L = 0;
+    int prec = 8; // bytes per real component
+    int maxe = 16;
+    uint32_t tpi = 0, nstages = 0, inst_stride = 0;
+    KfaStage st[KFA_MAX_STAGES] = {};
+    DevBuf tw; // stage twiddles
+
+    bool plan(uint32_t L, int prec);
+    uint32_t pick_ipw(bool jfast) const;
+    uint32_t lds_bytes(uint32_t ipw) const;
+};
+
+// Radix list for L (empty when L has a prime factor > 31).
+std::vector<uint32_t> radix_list(uint32_t L);
+bool lds_capable(uint32_t L, int prec);
+
+// A smooth length executed as one LDS pass or a four-step pair of passes.
+struct SubFft {
+    uint32_t L = 0;
+    int prec = 8;
+    bool four = false;
+    LdsPass direct;
+    uint32_t N1 = 0, N2 = 0; // four-step: column length N1, row length N2
+    LdsPass col, row;
+    DevBuf tw4_hi, tw4_lo;
+    uint32_t tw4_shift = 0;
+
+    bool plan(uint32_t L, int prec, bool force_four = false);
+    // scratch (in complex elements) needed for `batch` instances
+    size_t scratch_elems(uint64_t batch) const { return four ? (size_t)batch * L : 0; }
+    // Run with fully described first-load and last-store descriptors.  For
+    // four-step the load/store must have C == 1 (contiguous instances).
+    hipError_t run(const KfaIo& load, const KfaIo& store, double scale, double scale_post,
+                   uint64_t batch, void* scratch, hipStream_t s) const;
+};
+
+bool subfft_capable(uint32_t L, int prec);
+
+enum class Kind { Identity, Direct, Rader, Bluestein };
+
+class Plan1D {
+  public:
+    Plan1D(uint32_t n, int prec) : n_(n), prec_(prec) {}
+    bool build();
+    uint32_t n() const { return n_; }
+    int prec() const { return prec_; }
+    Kind kind() const { return kind_; }
+    const char* describe() const;
+
+    // scratch bytes for a batch of `batch` instances
+    size_t scratch_bytes(uint64_t batch) const;
+    // Execute `batch` transforms.  `scale` multiplies every output (1/N for a
+    // normalised inverse).  Column layouts (C > 1) are supported by Direct and
+    // by Rader/Bluestein whose inner length is Direct.
+    hipError_t execute(const Layout& in, const Layout& out, uint64_t batch, bool inverse,
+                       double scale, void* scratch, hipStream_t s, const Fuse* fuse = nullptr) const;
+    bool supports_columns() const;
+
+  private:
+    uint32_t n_;
+    int prec_;
+    Kind kind_ = Kind::Identity;
+    SubFft sub_;  // Direct: the transform; Rader: length n-1; Bluestein: length M
+    uint32_t M_ = 0;
+    DevBuf qidx_, pidx_, bhat_; // Rader
+    DevBuf chirp_, hhat_;       // Bluestein
+    mutable std::string desc_;
+};
+
+// -------------------------------------------------------------- plan registry
+// Plans are cached by (n, precision) and shared; they are immutable after build.
+std::shared_ptr<Plan1D> get_plan(uint32_t n, int prec);
+
+// Super twiddles of the real-input transform: st[i] = exp(-+ i pi ((i+1)/h + 1/2)).
+bool make_real_twiddles(DevBuf& out, uint32_t h, int prec, bool inverse);
+
+size_t elem_bytes(int prec); // bytes of one complex value
+
+} // namespace kfa

@@ -1,0 +1,157 @@
+// The batched LDS-resident FFT pass, templated on precision and on the most
+// complex values a thread holds per stage (MAXE).
+//
+// A workgroup owns `ipw` consecutive instances of a length-L transform.
+//  1. Cooperative load: the workgroup walks its ipw*L elements, consecutive
+//     threads taking consecutive elements of one instance (row passes) or the
+//     same element of consecutive instances (in_jfast, column passes), applies
+//     the pass's fused load operations and writes the values into a padded LDS
+//     region per instance.
+//  2. Radix stages, LDS to LDS, `tpi` threads per instance (Stockham autosort):
+//       stage s of radix R, ns = product of earlier radices, butterfly b < L/R:
+//         x_r = lds[b + r L/R] * W_{ns R}^{r (b mod ns)}
+//         y   = DFT_R(x)
+//         lds[(b - b mod ns) R + (b mod ns) + r ns] = y_r
+//     A thread reads all of its butterflies, the workgroup synchronises, then
+//     it computes and writes them back.
+//  3. Cooperative store with the fused store operations, ordered like the load
+//     by out_jfast.
+// The DFT computed is always the forward one (negative exponent); inverse
+// transforms conjugate on the way in and out.
+#pragma once
+#include "kfa_device.h"
+
+namespace kfa {
+
+template <typename T, int R, int MAXE>
+__device__ __forceinline__ void lds_stage(const KfaFftArgs& A, uint32_t s, cx<T>* my, uint32_t lane,
+                                          bool act) {
+    constexpr int NB = (MAXE / R) > 0 ? (MAXE / R) : 1;
+    const uint32_t nb = A.L / R, ns = A.st[s].ns, tpi = A.tpi;
+    cx<T> v[NB][R];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        const uint32_t b = lane + i * tpi;
+        if (act && b < nb) {
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                v[i][r] = my[lds_index<T>(b + r * nb)];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        const uint32_t b = lane + i * tpi;
+        if (act && b < nb) {
+            const uint32_t k = (ns == 1) ? 0u : (b % ns);
+            if (ns > 1) {
+                const cx<T>* tw = (const cx<T>*)A.tw + A.st[s].tw_off + k * (R - 1);
+#pragma unroll
+                for (int r = 1; r < R; ++r)
+                    v[i][r] = cmul(v[i][r], tw[r - 1]);
+            }
+            dft<T, R>(v[i]);
+            const uint32_t base = (b - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                my[lds_index<T>(base + r * ns)] = v[i][r];
+        }
+    }
+    __syncthreads();
+}
+
+#define KFA_STAGE_CASE(R)                                                                         \
+    case R:                                                                                       \
+        if constexpr (R <= MAXE)                                                                  \
+            lds_stage<T, R, MAXE>(A, s, my, lane, act);                                           \
+        break;
+
+template <typename T, int MAXE>
+__global__ void __launch_bounds__(MAXE <= 16 ? 512 : 256) k_fft(const KfaFftArgs A) {
+    extern __shared__ __align__(16) unsigned char kfa_smem[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(kfa_smem);
+    const uint32_t tid = threadIdx.x, nthr = blockDim.x;
+    const uint64_t first = (uint64_t)blockIdx.x * A.ipw;
+    const uint32_t nin =
+        (uint32_t)((A.batch - first) < (uint64_t)A.ipw ? (A.batch - first) : (uint64_t)A.ipw);
+    const uint32_t L = A.L, total = nin * L;
+
+    // 1. cooperative load
+    for (uint32_t idx = tid; idx < total; idx += nthr) {
+        uint32_t j, e;
+        if (A.in_jfast) {
+            j = idx % nin;
+            e = idx / nin;
+        } else {
+            j = idx / L;
+            e = idx - j * L;
+        }
+        const Ctx c = make_ctx(A.in, first + j);
+        lds[(size_t)j * A.inst_stride + lds_index<T>(e)] = load_elem<T>(A.in, c, e);
+    }
+    __syncthreads();
+
+    // 2. radix stages
+    const uint32_t il = tid / A.tpi, lane = tid - il * A.tpi;
+    const bool act = il < nin;
+    cx<T>* my = lds + (size_t)(act ? il : 0) * A.inst_stride;
+    for (uint32_t s = 0; s < A.nstages; ++s) {
+        switch (A.st[s].radix) {
+            KFA_STAGE_CASE(2)
+            KFA_STAGE_CASE(3)
+            KFA_STAGE_CASE(4)
+            KFA_STAGE_CASE(5)
+            KFA_STAGE_CASE(7)
+            KFA_STAGE_CASE(8)
+            KFA_STAGE_CASE(9)
+            KFA_STAGE_CASE(11)
+            KFA_STAGE_CASE(13)
+            KFA_STAGE_CASE(16)
+            KFA_STAGE_CASE(17)
+            KFA_STAGE_CASE(19)
+            KFA_STAGE_CASE(23)
+            KFA_STAGE_CASE(29)
+            KFA_STAGE_CASE(31)
+        default:
+            break;
+        }
+    }
+
+    // 3. cooperative store
+    const bool addx0 = (A.out.flags & KFA_IO_ADDX0) != 0;
+    for (uint32_t idx = tid; idx < total; idx += nthr) {
+        uint32_t j, e;
+        if (A.out_jfast) {
+            j = idx % nin;
+            e = idx / nin;
+        } else {
+            j = idx / L;
+            e = idx - j * L;
+        }
+        const Ctx c = make_ctx(A.out, first + j);
+        const cx<T> x0 = addx0 ? load_aux<T>(A.out, c) : cx<T>{T(0), T(0)};
+        store_elem<T>(A, c, e, lds[(size_t)j * A.inst_stride + lds_index<T>(e)], x0);
+    }
+}
+
+template <typename T, int MAXE>
+static hipError_t launch_fft_impl(const KfaFftArgs& a, uint32_t lds_bytes, hipStream_t s) {
+    static bool attr_done = false;
+    if (!attr_done) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_fft<T, MAXE>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess)
+            return e;
+        attr_done = true;
+    }
+    const uint64_t groups = (a.batch + a.ipw - 1) / a.ipw;
+    if (groups == 0)
+        return hipSuccess;
+    if (groups > 0x7fffffffull)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_fft<T, MAXE>), dim3((unsigned)groups), dim3(a.tpi * a.ipw), lds_bytes, s,
+                       a);
+    return hipGetLastError();
+}
+
+} // namespace kfa

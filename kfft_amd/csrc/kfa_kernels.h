@@ -1,0 +1,43 @@
+// Host launchers of the gfx950 kernels (kfa_kernels.hip).  `prec` is the
+// size in bytes of one real component: 8 (double) or 4 (float).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kfa_common.h"
+
+namespace kfa {
+
+hipError_t launch_fft(int prec, int maxe, const KfaFftArgs& a, uint32_t lds_bytes, hipStream_t s);
+hipError_t launch_fft_f64(int maxe, const KfaFftArgs& a, uint32_t lds_bytes, hipStream_t s);
+hipError_t launch_fft_f32(int maxe, const KfaFftArgs& a, uint32_t lds_bytes, hipStream_t s);
+hipError_t launch_stage_twiddles(int prec, void* tw, const KfaFftArgs& a, hipStream_t s);
+hipError_t launch_twiddle_table(void* out_d, uint64_t count, uint64_t step, uint64_t M,
+                                hipStream_t s);
+hipError_t launch_twiddle_indexed(void* out_d, const uint32_t* idx, uint64_t count, uint64_t M,
+                                  int conjugate, hipStream_t s);
+hipError_t launch_bluestein_tables(void* chirp_d, void* h_d, uint64_t N, uint64_t M,
+                                   hipStream_t s);
+hipError_t launch_convert_from_f64(int prec, void* dst, const void* src_d, uint64_t n,
+                                   double scale, hipStream_t s);
+hipError_t launch_real_post(int prec, const void* Z, int64_t z_dist, void* X, int64_t xdh,
+                            int64_t xdl, int64_t xs, uint32_t C, uint32_t h, uint64_t batch,
+                            const void* st, int expand, int conj_out, hipStream_t s);
+hipError_t launch_real_pre(int prec, const void* X, int64_t xdh, int64_t xdl, int64_t xs,
+                           uint32_t C, void* Z, int64_t z_dist, uint32_t h, uint64_t batch,
+                           const void* st, int conj_in, hipStream_t s);
+hipError_t launch_transpose(int elem_bytes, const void* in, void* out, uint32_t x, uint32_t y,
+                            uint32_t nmat, hipStream_t s);
+hipError_t launch_hadamard(int prec, void* acc, const void* other, uint64_t n, hipStream_t s);
+hipError_t launch_hadamard_real(int prec, void* acc, const void* other, uint64_t n,
+                                hipStream_t s);
+hipError_t launch_copy(int prec, const void* in, void* out, uint64_t n, int conj_, double scale,
+                       hipStream_t s);
+hipError_t launch_gather_first(int prec, int real_in, const void* in, int64_t dist_hi,
+                               int64_t dist_lo, uint32_t C, void* out, uint64_t batch,
+                               hipStream_t s);
+
+hipError_t launch_rader_dc(int prec, const void* x0, int x0_real, const void* A, uint32_t L,
+                           void* out, int64_t dh, int64_t dl, uint32_t C, int real_out, int conj_,
+                           double scale, uint64_t batch, hipStream_t s);
+
+} // namespace kfa

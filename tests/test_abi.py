@@ -1,0 +1,142 @@
+"""CPU checks of the C-ABI libraries: loading, exported symbols, struct layout,
+service functions, pool / lenmem semantics and host-side helpers.  No GPU is
+needed; the plan-creation paths that need one are exercised in test_gpu_*.py.
+"""
+import ctypes as C
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+import kfft_amd as K
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+u8, u32, vp = C.c_uint8, C.c_uint32, C.c_void_p
+
+
+class PlanRader(C.Structure):
+    _fields_ = [("prime", u32), ("p", u32), ("q", u32), ("qidx", vp), ("pidx", vp),
+                ("shuffle_twiddles", vp), ("plan", vp), ("plan_inv", vp)]
+
+
+class PlanCpx(C.Structure):
+    """Mirror of kfft_plan_cpx (include/kfft_cpx.h; reference incs/kfft_cpx.h:46-65)."""
+    _fields_ = [("mmgr", vp), ("nfft", u32), ("level", u8), ("flags", u32), ("fac_count", u8),
+                ("factors", u32 * 64), ("prm_count", u8), ("primes", PlanRader * 32),
+                ("twiddles", vp), ("gpu", vp)]
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        for m in re.finditer(r"KFFT_API\s+[^;{]*?\b(kfft_\w+)\s*\(", text, re.S):
+            names.add(m.group(1))
+    return names
+
+
+@pytest.mark.parametrize("half", [False, True])
+def test_library_loads_and_exports_declared_symbols(half):
+    lib = K.load(half)
+    declared = declared_symbols()
+    assert len(declared) >= 50
+    missing = [s for s in sorted(declared) if not hasattr(lib, s)]
+    assert not missing, missing
+    assert set(K.EXPORTS) <= declared
+
+
+@pytest.mark.parametrize("half", [False, True])
+def test_info_and_service(half):
+    ver, flags = K.info(half)
+    assert ver == (0, 8, 1)
+    assert flags & (1 << 5)  # KFFT_INFO_RADER_ALGO
+    assert bool(flags & (1 << 7)) == half  # KFFT_INFO_HALF_SCALAR
+    lib = K.load(half)
+    assert lib.kfft_strerr(0) == b"Success operation result"
+    assert lib.kfft_strerr(4) == b"Use improper real function for this plan"
+
+    def smooth(m):
+        for q in (2, 3, 5):
+            while m % q == 0:
+                m //= q
+        return m == 1
+
+    for n in [1, 7, 11, 97, 1009, 4097, 65537]:
+        assert lib.kfft_next_fast_size(n) == next(m for m in range(n, 4 * n) if smooth(m))
+
+
+def test_plan_struct_layout_and_lenmem():
+    lib = K.load(False)
+    need = C.c_size_t(0)
+    assert lib.kfft_config_cpx(0, 0, None, C.byref(need)) is None
+    # test/unit/suites/api_cpx1.c:26-36: the size query for N=0 is sizeof(kfft_plan_cpx)
+    assert need.value == C.sizeof(PlanCpx)
+    need101 = C.c_size_t(0)
+    lib.kfft_config_cpx(101, 0, None, C.byref(need101))
+    assert need101.value == C.sizeof(PlanCpx) + 101 * 16  # plan plus its twiddle table
+    # a pool given with too little room reports the need and returns NULL (incs/kfft_algo.h:27-35)
+    pool = lib.kfft_pool_create(64)
+    small = C.c_size_t(8)
+    assert lib.kfft_config_cpx(101, 0, pool, C.byref(small)) is None
+    assert small.value == need101.value
+    lib.kfft_pool_free(pool)
+    # the float build's complex type is 8 bytes
+    lh = K.load(True)
+    needh = C.c_size_t(0)
+    lh.kfft_config_cpx(101, 0, None, C.byref(needh))
+    assert needh.value - C.sizeof(PlanCpx) == 101 * 8
+
+
+def test_pool_semantics():
+    """Bump allocator of kfft_pool.c:13-96 (as pinned by test/unit/suites/sys_pool.c)."""
+    lib = K.load(False)
+    A = lib.kfft_pool_create(100)
+    assert lib.kfft_pool_empty(A) == 100
+    p1 = lib.kfft_pool_alloc(A, 60)
+    p2 = lib.kfft_pool_alloc(A, 40)
+    assert p1 and p2 and p2 - p1 == 60
+    assert lib.kfft_pool_empty(A) == 0
+    assert lib.kfft_pool_alloc(A, 1) is None
+    assert lib.kfft_pool_alloc(A, 0) is None
+    lib.kfft_pool_clear(A)
+    assert lib.kfft_pool_empty(A) == 100
+    assert lib.kfft_pool_alloc(A, 100) == p1
+    lib.kfft_pool_free(A)
+
+
+def test_cleanup_null():
+    lib = K.load(False)
+    obj = (vp * 1)(None)
+    assert lib.kfft_cleanup(C.cast(obj, vp)) == K.RET_FREE_NULL
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 8, 9])
+def test_shift_host(n):
+    """kfft_shift_cpx/_scalar (kfft_shift.c:4-61) equal numpy's fftshift / ifftshift."""
+    lib = K.load(False)
+    x = np.arange(n, dtype=np.float64) + 0.5
+    for inverse, ref in [(False, np.fft.fftshift(x)), (True, np.fft.ifftshift(x))]:
+        b = x.copy()
+        lib.kfft_shift_scalar(C.c_void_p(b.ctypes.data), n, inverse, None)
+        assert np.array_equal(b, ref)
+        c = (x + 1j * x).copy()
+        lib.kfft_shift_cpx(C.c_void_p(c.ctypes.data), n, inverse, None)
+        assert np.array_equal(c, ref + 1j * ref)
+
+
+def test_shift2_and_math_host():
+    lib = K.load(False)
+    y, x = 5, 6
+    a = (np.arange(x * y) + 1j * np.arange(x * y)[::-1]).reshape(y, x).astype(np.complex128)
+    b = a.copy()
+    lib.kfft_shift2_cpx(C.c_void_p(b.ctypes.data), None, x, y, False, None)
+    assert np.array_equal(b, np.fft.fftshift(a))
+    o = np.empty(x * y, np.complex128)
+    lib.kfft_math_transpose_cpx(C.c_void_p(a.ctypes.data), C.c_void_p(o.ctypes.data), x, y)
+    assert np.array_equal(o.reshape(x, y), a.T)
+    h = a.copy()
+    lib.kfft_math_hadamard_cpx(C.c_void_p(h.ctypes.data), C.c_void_p(a.ctypes.data), x * y)
+    assert np.allclose(h, a * a, rtol=1e-15)

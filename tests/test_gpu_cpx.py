@@ -1,0 +1,176 @@
+"""GPU parity of the 1D complex transform (kfft_config_cpx / kfft_eval_cpx and
+the batched entry points) against the CPU oracle.
+
+Bar: relative L2 error <= 1e-12 * log2(N) for the double build (the plain
+form), on seeded uniform[-1,1) inputs.  Lengths cover every plan shape:
+one LDS pass, four-step, Rader (prime N with smooth N-1) and Bluestein.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import kfft_amd as K
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))
+
+
+def tol(n):
+    return 1e-12 * max(1.0, math.log2(n))
+
+
+def cplx(rng, *shape):
+    return rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)
+
+
+# direct / four-step / rader / bluestein; the reference factors 121 and 169
+# wrongly (see tests/test_oracle.py), so those compare with numpy instead
+LENGTHS = [1, 2, 3, 4, 5, 7, 8, 9, 11, 13, 16, 17, 25, 31, 32, 37, 60, 64, 97, 100, 128, 210, 256,
+           330, 512, 1000, 1008, 1009, 1024, 2048, 2018, 4096, 4160, 6561, 8192, 10000, 12289,
+           16384, 65536, 65537, 94132]
+
+
+@pytest.mark.parametrize("n", LENGTHS)
+def test_forward_inverse_vs_oracle(n):
+    rng = np.random.default_rng(1000 + n)
+    batch = 3 if n <= 70000 else 1
+    x = cplx(rng, batch, n)
+    fwd = K.CpxPlan(n)
+    X = fwd(x)
+    assert rel(X, O.fft(x)) <= tol(n), fwd.describe()
+    inv = K.CpxPlan(n, K.FLAG_INVERSE)
+    assert rel(inv(X), O.fft(X, O.FLAG_INVERSE)) <= tol(n), inv.describe()
+    raw = K.CpxPlan(n, K.FLAG_INVERSE | K.FLAG_DISABLE_NORM)
+    assert rel(raw(X), O.fft(X, O.FLAG_INVERSE | O.FLAG_DISABLE_NORM)) <= tol(n)
+
+
+@pytest.mark.parametrize("n", [121, 169])
+def test_prime_squares_are_true_dfts(n):
+    """Where the reference's factoriser fails (N = p^2, bisection sqrt below p),
+    the drop-in still returns the DFT; compare with numpy and with the oracle's
+    GENERIC path (which does not take the broken Rader branch)."""
+    rng = np.random.default_rng(n)
+    x = cplx(rng, n)
+    X = K.CpxPlan(n)(x)
+    assert rel(X, np.fft.fft(x)) <= tol(n)
+    assert rel(X, O.fft(x, O.FLAG_GENERIC)) <= tol(n)
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 7, 330])
+def test_golden_vectors(n):
+    """test/unit/suites/defs_cpx1.h through the GPU at the reference's 1e-3."""
+    g = json.load(open(os.path.join(HERE, "golden", "defs_cpx1.json")))[str(n)]
+    xin = np.array(g["in"]) @ np.array([1, 1j])
+    xout = np.array(g["out"]) @ np.array([1, 1j])
+    assert np.abs(K.CpxPlan(n)(xin) - xout).max() < 1e-3
+    assert np.abs(K.CpxPlan(n, K.FLAG_INVERSE)(xout) - xin).max() < 1e-3
+
+
+@pytest.mark.parametrize("flags", [K.FLAG_GENERIC, K.FLAG_GENERIC_ONLY])
+def test_generic_flags_same_dft(flags):
+    rng = np.random.default_rng(5)
+    x = cplx(rng, 330)
+    assert rel(K.CpxPlan(330, flags)(x), O.fft(x, flags)) <= tol(330)
+
+
+@pytest.mark.parametrize("n", [64, 4096, 1009, 65536])
+def test_device_pointers_and_in_place(n):
+    import torch
+
+    rng = np.random.default_rng(n + 7)
+    x = cplx(rng, 4, n)
+    ref = O.fft(x)
+    p = K.CpxPlan(n)
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.empty_like(xd)
+    p(xd, out=yd)
+    assert rel(yd.cpu().numpy(), ref) <= tol(n)
+    p(xd, out=xd)  # fin == fout (kfft_cpx.c:329-345 allows it)
+    assert rel(xd.cpu().numpy(), ref) <= tol(n)
+    # asynchronous batched entry point on the library stream
+    zd = torch.from_numpy(x).cuda()
+    od = torch.empty_like(zd)
+    p.run_async(zd, od, 4)
+    K.synchronize()
+    assert rel(od.cpu().numpy(), ref) <= tol(n)
+
+
+def test_batch_equals_single():
+    rng = np.random.default_rng(9)
+    x = cplx(rng, 7, 1000)
+    p = K.CpxPlan(1000)
+    out = p(x)
+    for b in range(7):
+        assert np.array_equal(out[b], p(x[b]))
+
+
+@pytest.mark.parametrize("n", [4096, 1 << 16, 1 << 20])
+def test_batch_sum_matches_fft_of_sum(n):
+    """Size-independent check at large batch: linearity of the whole batch
+    (sum of transforms == transform of the sum) against one oracle transform,
+    plus the forward/inverse round trip on every instance."""
+    import torch
+
+    batch = max(4, (64 << 20) // (16 * n))
+    g = torch.Generator(device="cuda").manual_seed(n)
+    xd = (torch.rand(batch, n, 2, generator=g, device="cuda", dtype=torch.float64) * 2 - 1)
+    xd = torch.view_as_complex(xd).contiguous()
+    yd = torch.empty_like(xd)
+    K.CpxPlan(n)(xd, out=yd)
+    s_in = xd.sum(0).cpu().numpy()
+    s_out = yd.sum(0).cpu().numpy()
+    assert rel(s_out, O.fft(s_in)) <= tol(n) * 4
+    zd = torch.empty_like(xd)
+    K.CpxPlan(n, K.FLAG_INVERSE)(yd, out=zd)
+    err = (torch.linalg.vector_norm(zd - xd) / torch.linalg.vector_norm(xd)).item()
+    assert err <= tol(n)
+
+
+def test_plan_pool_semantics():
+    """test/unit/suites/api_cpx1.c:12-102 on the drop-in: plan release, two plans
+    in one user pool, overflow returns NULL."""
+    import ctypes as C
+
+    lib = K.load(False)
+    p = lib.kfft_config_cpx(1, 0, None, None)
+    assert p
+    assert lib.kfft_cleanup(p) == 0
+    s1, s2 = C.c_size_t(0), C.c_size_t(0)
+    lib.kfft_config_cpx(101, K.FLAG_INVERSE, None, C.byref(s1))
+    lib.kfft_config_cpx(78, 0, None, C.byref(s2))
+    pool = lib.kfft_pool_create(s1.value + s2.value)
+    a = lib.kfft_config_cpx(78, 0, pool, None)
+    b = lib.kfft_config_cpx(101, K.FLAG_INVERSE, pool, None)
+    assert a and b
+    lib.kfft_pool_free(pool)
+    pool = lib.kfft_pool_create(s1.value + s2.value)
+    a = lib.kfft_config_cpx(78, 0, pool, None)
+    b = lib.kfft_config_cpx(107, K.FLAG_INVERSE, pool, None)
+    assert a and not b
+    # RENEW clears the pool and builds the new plan in it
+    c = lib.kfft_config_cpx(101, K.FLAG_INVERSE | K.FLAG_RENEW, pool, None)
+    assert c
+    rng = np.random.default_rng(1)
+    x = cplx(rng, 101)
+    out = np.empty_like(x)
+    assert lib.kfft_eval_cpx(c, C.c_void_p(x.ctypes.data), C.c_void_p(out.ctypes.data)) == 0
+    assert rel(out, O.fft(x, O.FLAG_INVERSE)) <= tol(101)
+    lib.kfft_pool_free(pool)
+
+
+@pytest.mark.parametrize("n", [256, 4096, 1009, 65536])
+def test_float_build_plain(n):
+    """The KFFT_HALF_SCALAR (float) build: FP32 arithmetic, FP64-generated twiddles."""
+    rng = np.random.default_rng(n + 3)
+    x = cplx(rng, 2, n).astype(np.complex64)
+    X = K.CpxPlan(n, half=True)(x)
+    ref = O.fft(x.astype(np.complex128))
+    assert rel(X, ref) <= 2e-6 * max(1, math.log2(n)) / 4
