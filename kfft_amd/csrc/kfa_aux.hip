@@ -60,6 +60,16 @@ __global__ void k_twiddle_table(cx<double>* out, uint64_t count, uint64_t step, 
         out[i] = twiddle_d(i * step, M);
 }
 
+// tab[k * cols + c] = exp(-2 pi i * (k c) / M): the four-step twiddles laid out so
+// that consecutive columns c are adjacent
+__global__ void k_twiddle_2d(cx<double>* out, uint64_t rows, uint64_t cols, uint64_t M) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < rows * cols) {
+        const uint64_t k = i / cols, c = i - k * cols;
+        out[i] = twiddle_d(k * c, M);
+    }
+}
+
 // tab[i] = exp(-2 pi i * idx[i] / M)  (Rader b vector)
 __global__ void k_twiddle_indexed(cx<double>* out, const uint32_t* idx, uint64_t count,
                                   uint64_t M, int conjugate) {
@@ -282,6 +292,16 @@ hipError_t launch_twiddle_table(void* out_d, uint64_t count, uint64_t step, uint
         return hipSuccess;
     hipLaunchKernelGGL(k_twiddle_table, dim3(blocks_for(count, 256)), dim3(256), 0, s,
                        (cx<double>*)out_d, count, step, M);
+    return hipGetLastError();
+}
+
+hipError_t launch_twiddle_2d(void* out_d, uint64_t rows, uint64_t cols, uint64_t M,
+                             hipStream_t s) {
+    const uint64_t n = rows * cols;
+    if (!n)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_twiddle_2d, dim3(blocks_for(n, 256)), dim3(256), 0, s,
+                       (cx<double>*)out_d, rows, cols, M);
     return hipGetLastError();
 }
 

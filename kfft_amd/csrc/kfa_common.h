@@ -68,4 +68,27 @@ struct KfaFftArgs {
 // LDS padding: one spare element every 2^KFA_PAD_SHIFT elements keeps the
 // stride-R writes of a Stockham stage off a single bank group.
 #define KFA_PAD_SHIFT_F64 4
-#define KFA_PAD_SHIFT_F32 5
+#define KFA_PAD_SHIFT_F32 4
+
+// Arguments of the compile-time shaped power-of-two passes (kfa_fast_impl.h).
+#define KFA_FAST_CONJ_IN 1u
+#define KFA_FAST_CONJ_OUT 2u
+#define KFA_FAST_SCALE 4u
+#define KFA_FAST_TW4 8u
+#define KFA_FAST_NT_IN 16u  // non-temporal (streaming) loads of the input
+#define KFA_FAST_NT_OUT 32u // non-temporal (streaming) stores of the output
+
+struct FastArgs {
+    const void* in;
+    void* out;
+    const void* tw;  // stage twiddles (same layout as LdsPass)
+    const void* tw4; // tw4[k * out_C + col] = W_N^(col * k), compute precision (pass A)
+    int64_t in_hi, in_lo, in_stride;
+    int64_t out_hi, out_lo, out_stride;
+    uint64_t batch;
+    uint32_t in_C, out_C;
+    uint32_t flags;
+    uint32_t pad;
+    double scale;
+};
+

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 
@@ -274,7 +275,59 @@ bool SubFft::plan(uint32_t len, int p, bool force_four) {
     if (!hip_ok(launch_twiddle_table(tw4_lo.p, S, 1, L, s), "tw4 lo") ||
         !hip_ok(launch_twiddle_table(tw4_hi.p, nhi, S, L, s), "tw4 hi"))
         return false;
+    if (fast_len(N1) && fast_len(N2)) {
+        DevBuf full64;
+        if (!full64.alloc((size_t)L * 16) || !tw4_full.alloc((size_t)L * elem_bytes(prec)))
+            return false;
+        if (!hip_ok(launch_twiddle_2d(full64.p, N1, N2, L, s), "tw4 full") ||
+            !hip_ok(launch_convert_from_f64(prec, tw4_full.p, full64.p, L, 1.0, s), "tw4 cvt") ||
+            !hip_ok(hipStreamSynchronize(s), "tw4 sync"))
+            return false;
+    }
     return true;
+}
+
+// Tuning of the two-pass power-of-two path, fixed per process.
+//  KFFT_AMD_CHUNK_MB: run the two passes over groups of transforms whose
+//                     intermediate fits this many MiB, so the second pass can
+//                     read part of it back from the caches (0: whole batch)
+//  KFFT_AMD_NT:       non-temporal access bits: 1 pass-A input, 2 pass-B output,
+//                     4 pass-A output, 8 pass-B input
+struct Tuning {
+    // defaults measured on MI355X for 2^16 x 4096 (FP32 and FP64)
+    size_t chunk_bytes = (size_t)256 << 20;
+    uint32_t nt = 3;
+};
+static const Tuning& tuning() {
+    static const Tuning t = [] {
+        Tuning v;
+        if (const char* c = getenv("KFFT_AMD_CHUNK_MB"))
+            v.chunk_bytes = (size_t)(atof(c) * (1u << 20));
+        if (const char* n = getenv("KFFT_AMD_NT"))
+            v.nt = (uint32_t)atoi(n);
+        return v;
+    }();
+    return t;
+}
+
+// The fast passes fuse only conjugation (load / store) and scaling (store).
+static bool fast_io_ok(const KfaIo& io, uint32_t allowed) {
+    return (io.flags & ~allowed) == 0 && io.valid == 0xffffffffu && io.lo_mul == 0 &&
+           io.e_mul == 1;
+}
+
+static hipError_t launch_fast(int prec, uint32_t L, int mode, const FastArgs& a, hipStream_t s) {
+    return prec == 8 ? launch_fast_f64(L, mode, a, s) : launch_fast_f32(L, mode, a, s);
+}
+
+static FastArgs fast_args(const void* tw, uint64_t batch, double scale) {
+    FastArgs f;
+    memset(&f, 0, sizeof(f));
+    f.tw = tw;
+    f.batch = batch;
+    f.scale = scale;
+    f.in_C = f.out_C = 1;
+    return f;
 }
 
 static bool io_jfast(const KfaIo& io) {
@@ -311,12 +364,96 @@ static hipError_t run_pass(const LdsPass& P, const KfaIo& in, const KfaIo& out, 
     return launch_fft(P.prec, P.maxe, a, P.lds_bytes(a.ipw), s);
 }
 
+// Both passes of the fast two-pass transform for `batch` contiguous instances.
+hipError_t SubFft::run_four_fast(const KfaIo& load, const KfaIo& store, double scale,
+                                 uint64_t batch, void* scratch, uint32_t nt,
+                                 hipStream_t s) const {
+    // pass A: columns n2 of length N1 (stride N2), W_L^(n2 k1) on store
+    FastArgs a = fast_args(col.tw.p, batch * N2, 1.0);
+    a.in = (const void*)load.ptr;
+    a.in_C = N2;
+    a.in_hi = load.dist_hi;
+    a.in_lo = load.stride;
+    a.in_stride = (int64_t)N2 * load.stride;
+    a.out = scratch;
+    a.out_C = N2;
+    a.out_hi = L;
+    a.out_lo = 1;
+    a.out_stride = N2;
+    a.tw4 = tw4_full.p;
+    a.flags = KFA_FAST_TW4 | ((load.flags & KFA_IO_CONJ1) ? KFA_FAST_CONJ_IN : 0) |
+              ((nt & 1) ? KFA_FAST_NT_IN : 0) | ((nt & 4) ? KFA_FAST_NT_OUT : 0);
+    hipError_t e = launch_fast(prec, N1, 1, a, s);
+    if (e != hipSuccess)
+        return e;
+    // pass B: rows k1 of length N2, natural-order store X[k1 + N1 k2]
+    FastArgs b = fast_args(row.tw.p, batch * N1, scale);
+    b.in = scratch;
+    b.in_C = N1;
+    b.in_hi = L;
+    b.in_lo = N2;
+    b.in_stride = 1;
+    b.out = (void*)store.ptr;
+    b.out_C = N1;
+    b.out_hi = store.dist_hi;
+    b.out_lo = store.stride;
+    b.out_stride = (int64_t)N1 * store.stride;
+    b.flags = ((store.flags & KFA_IO_CONJ2) ? KFA_FAST_CONJ_OUT : 0) |
+              ((store.flags & KFA_IO_SCALE) ? KFA_FAST_SCALE : 0) |
+              ((nt & 2) ? KFA_FAST_NT_OUT : 0) | ((nt & 8) ? KFA_FAST_NT_IN : 0);
+    return launch_fast(prec, N2, 2, b, s);
+}
+
 hipError_t SubFft::run(const KfaIo& load, const KfaIo& store, double scale, double scale_post,
                        uint64_t batch, void* scratch, hipStream_t s) const {
-    if (!four)
+    const bool simple = fast_io_ok(load, KFA_IO_CONJ1) &&
+                        fast_io_ok(store, KFA_IO_CONJ2 | KFA_IO_SCALE) && scale_post == 1.0;
+    if (!four) {
+        if (simple && fast_len(L)) {
+            const bool rows = load.C == 1 && store.C == 1 && load.stride == 1 && store.stride == 1;
+            const bool cols = load.C == store.C && load.dist_lo == 1 && store.dist_lo == 1 &&
+                              load.stride != 1 && store.stride != 1;
+            if (rows || cols) {
+                FastArgs f = fast_args(direct.tw.p, batch, scale);
+                f.in = (const void*)load.ptr;
+                f.out = (void*)store.ptr;
+                f.in_C = load.C;
+                f.in_hi = load.dist_hi;
+                f.in_lo = load.dist_lo;
+                f.in_stride = load.stride;
+                f.out_C = store.C;
+                f.out_hi = store.dist_hi;
+                f.out_lo = store.dist_lo;
+                f.out_stride = store.stride;
+                f.flags = ((load.flags & KFA_IO_CONJ1) ? KFA_FAST_CONJ_IN : 0) |
+                          ((store.flags & KFA_IO_CONJ2) ? KFA_FAST_CONJ_OUT : 0) |
+                          ((store.flags & KFA_IO_SCALE) ? KFA_FAST_SCALE : 0);
+                return launch_fast(prec, L, rows ? 0 : 1, f, s);
+            }
+        }
         return run_pass(direct, load, store, scale, scale_post, batch, nullptr, nullptr, 0, 0, s);
+    }
     if (load.C != 1 || store.C != 1)
         return hipErrorInvalidValue;
+    if (simple && tw4_full.p) {
+        const Tuning& tn = tuning();
+        uint64_t chunk = batch;
+        if (tn.chunk_bytes) {
+            chunk = tn.chunk_bytes / ((size_t)L * elem_bytes(prec));
+            chunk = std::max<uint64_t>(1, std::min<uint64_t>(chunk, batch));
+        }
+        const int64_t esz = (int64_t)elem_bytes(prec);
+        for (uint64_t c0 = 0; c0 < batch; c0 += chunk) {
+            const uint64_t nb = std::min<uint64_t>(chunk, batch - c0);
+            KfaIo ld = load, st = store;
+            ld.ptr += (uint64_t)((int64_t)c0 * load.dist_hi * esz);
+            st.ptr += (uint64_t)((int64_t)c0 * store.dist_hi * esz);
+            hipError_t e = run_four_fast(ld, st, scale, nb, scratch, tn.nt, s);
+            if (e != hipSuccess)
+                return e;
+        }
+        return hipSuccess;
+    }
     // pass A: columns of length N1 (stride N2), twiddle W_L^(n2 k1) on store
     KfaIo ia = load;
     ia.C = N2;

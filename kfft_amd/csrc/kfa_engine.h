@@ -92,6 +92,7 @@ struct SubFft {
     LdsPass col, row;
     DevBuf tw4_hi, tw4_lo;
     uint32_t tw4_shift = 0;
+    DevBuf tw4_full; // W_L^m, m < L, compute precision (fast four-step passes)
 
     bool plan(uint32_t L, int prec, bool force_four = false);
     // scratch (in complex elements) needed for `batch` instances
@@ -100,6 +101,8 @@ struct SubFft {
     // four-step the load/store must have C == 1 (contiguous instances).
     hipError_t run(const KfaIo& load, const KfaIo& store, double scale, double scale_post,
                    uint64_t batch, void* scratch, hipStream_t s) const;
+    hipError_t run_four_fast(const KfaIo& load, const KfaIo& store, double scale, uint64_t batch,
+                             void* scratch, uint32_t nt, hipStream_t s) const;
 };
 
 bool subfft_capable(uint32_t L, int prec);

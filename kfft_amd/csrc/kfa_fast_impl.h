@@ -1,0 +1,326 @@
+// Compile-time shaped FFT passes for power-of-two lengths 16..4096.
+//
+// These carry the batched C2C hot path (one-pass transforms and both passes of
+// a four-step transform).  Every thread owns E = 16 complex values (8 for
+// L = 32): stage 0 loads them straight from HBM into registers, each stage
+// boundary exchanges them once through LDS, and the last stage stores straight
+// from registers to HBM.  Shapes, radices, thread counts and LDS strides are
+// template constants, so the only runtime index arithmetic is the instance
+// base address.
+//
+// Modes (thread -> work mapping on the load side / store side):
+//   ROW   instance-major on both sides (contiguous rows, lanes walk elements)
+//   COL   lanes walk instances on both sides (columns of a row-major matrix;
+//         pass A of four-step, optional W_N^(col*k) twiddle on store)
+//   TOUT  load like ROW, store like COL (pass B of four-step: the output
+//         X[k1 + N1*k2] is contiguous across consecutive instances k1)
+#pragma once
+#include <stdlib.h>
+
+#include "kfa_device.h"
+
+namespace kfa {
+
+enum { FAST_ROW = 0, FAST_COL = 1, FAST_TOUT = 2 };
+
+// radix sequence of radix_list() for L = 2^a: 16s, then 8 / 4 / 2 (or 8,4 for
+// a = 4q+1, q > 0)
+template <int L>
+struct P2 {
+    static constexpr int a = (L >= 4096) ? 12 : (L >= 2048) ? 11 : (L >= 1024) ? 10 : (L >= 512) ? 9
+                           : (L >= 256) ? 8 : (L >= 128) ? 7 : (L >= 64) ? 6 : (L >= 32) ? 5 : 4;
+    static constexpr int q = a / 4, rr = a % 4;
+    static constexpr int S = (rr == 0) ? q : (rr == 1 && q > 0) ? q + 1 : q + 1;
+    static constexpr int R(int s) {
+        if (rr == 1 && q > 0) {
+            if (s < q - 1)
+                return 16;
+            return s == q - 1 ? 8 : 4;
+        }
+        if (s < q)
+            return 16;
+        return rr == 1 ? 2 : rr == 2 ? 4 : 8;
+    }
+    static constexpr int NS(int s) { // product of the radices before stage s
+        int p = 1;
+        for (int i = 0; i < s; ++i)
+            p *= R(i);
+        return p;
+    }
+    static constexpr int TWOFF(int s) { // offset of stage s in the twiddle table
+        int o = 0;
+        for (int i = 1; i < s; ++i)
+            o += NS(i) * (R(i) - 1);
+        return o;
+    }
+    static constexpr int E = (L == 32) ? 8 : 16;
+    static constexpr int TPI = L / E;
+    static constexpr int PAD_L(int shift) { return L + (L >> shift); }
+};
+
+// streaming (non-temporal) element access, used to keep single-use input and
+// output traffic from displacing data that is read again from the caches
+template <typename T>
+struct VecOf;
+template <>
+struct VecOf<float> {
+    typedef float type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct VecOf<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+template <typename T>
+__device__ __forceinline__ cx<T> load_nt(const cx<T>* p) {
+    typedef typename VecOf<T>::type V;
+    const V v = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
+    return {v.x, v.y};
+}
+template <typename T>
+__device__ __forceinline__ void store_nt(cx<T>* p, cx<T> x) {
+    typedef typename VecOf<T>::type V;
+    V v;
+    v.x = x.re;
+    v.y = x.im;
+    __builtin_nontemporal_store(v, reinterpret_cast<V*>(p));
+}
+
+template <typename T, int L, int MODE, int IPW>
+__global__ void __launch_bounds__(IPW* P2<L>::TPI) k_fast(const FastArgs A) {
+    using S = P2<L>;
+    constexpr int E = S::E, TPI = S::TPI, NST = S::S;
+    constexpr int PSH = sizeof(T) == 8 ? KFA_PAD_SHIFT_F64 : KFA_PAD_SHIFT_F32;
+    constexpr int SP = (S::PAD_L(PSH)) | 1; // odd LDS stride per instance
+    extern __shared__ __align__(16) unsigned char kfa_smem[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(kfa_smem);
+
+    const uint32_t tid = threadIdx.x;
+    const uint64_t first = (uint64_t)blockIdx.x * IPW;
+    // load side mapping
+    const uint32_t li = (MODE == FAST_COL) ? tid % IPW : tid / TPI;
+    const uint32_t ll = (MODE == FAST_COL) ? tid / IPW : tid % TPI;
+    // store side (and middle stages) mapping
+    const uint32_t si = (MODE == FAST_ROW) ? tid / TPI : tid % IPW;
+    const uint32_t sl = (MODE == FAST_ROW) ? tid % TPI : tid / IPW;
+
+    cx<T> v[E];
+    // ---- stage 0: HBM -> registers, DFT_R0, registers -> LDS (or HBM)
+    {
+        constexpr int R = S::R(0), NB = E / R, STR = L / R;
+        const uint64_t inst = first + li;
+        const bool act = inst < A.batch;
+        const cx<T>* src = (const cx<T>*)A.in;
+        int64_t base = 0;
+        if (act) {
+            const uint64_t hi = inst / A.in_C, lo = inst - hi * A.in_C;
+            base = (int64_t)hi * A.in_hi + (int64_t)lo * A.in_lo;
+        }
+#pragma unroll
+        for (int i = 0; i < NB; ++i)
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int e = (int)ll + i * TPI + r * STR;
+                cx<T> x = {T(0), T(0)};
+                if (act) {
+                    const cx<T>* p = src + base + (int64_t)e * A.in_stride;
+                    x = (A.flags & KFA_FAST_NT_IN) ? load_nt(p) : *p;
+                }
+                if (A.flags & KFA_FAST_CONJ_IN)
+                    x.im = -x.im;
+                v[i * R + r] = x;
+            }
+#pragma unroll
+        for (int i = 0; i < NB; ++i)
+            dft<T, R>(v + i * R);
+        if constexpr (NST > 1) {
+            cx<T>* my = lds + (size_t)li * SP;
+#pragma unroll
+            for (int i = 0; i < NB; ++i) {
+                const int b = (int)ll + i * TPI;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int e = b * R + r;
+                    my[e + (e >> PSH)] = v[i * R + r];
+                }
+            }
+        }
+    }
+    // ---- stages 1..NST-1: LDS -> registers, twiddle, DFT, -> LDS or HBM
+#pragma unroll
+    for (int s = 1; s < NST; ++s) {
+        __syncthreads();
+        const int R = S::R(s), NS = S::NS(s), NB = E / R, STR = L / R;
+        cx<T>* my = lds + (size_t)si * SP;
+#pragma unroll
+        for (int i = 0; i < NB; ++i)
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int e = (int)sl + i * TPI + r * STR;
+                v[i * R + r] = my[e + (e >> PSH)];
+            }
+        const cx<T>* tw = (const cx<T>*)A.tw + S::TWOFF(s);
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const int b = (int)sl + i * TPI, k = b % NS;
+#pragma unroll
+            for (int r = 1; r < R; ++r)
+                v[i * R + r] = cmul(v[i * R + r], tw[k * (R - 1) + r - 1]);
+        }
+        // R is a compile-time constant after unrolling over s
+        if (R == 16) {
+#pragma unroll
+            for (int i = 0; i < E / 16; ++i)
+                dft<T, 16>(v + i * 16);
+        } else if (R == 8) {
+#pragma unroll
+            for (int i = 0; i < E / 8; ++i)
+                dft<T, 8>(v + i * 8);
+        } else if (R == 4) {
+#pragma unroll
+            for (int i = 0; i < E / 4; ++i)
+                dft<T, 4>(v + i * 4);
+        } else {
+#pragma unroll
+            for (int i = 0; i < E / 2; ++i)
+                dft<T, 2>(v + i * 2);
+        }
+        if (s + 1 < NST) {
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < NB; ++i) {
+                const int b = (int)sl + i * TPI, k = b % NS;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int e = (b - k) * R + k + r * NS;
+                    my[e + (e >> PSH)] = v[i * R + r];
+                }
+            }
+        }
+    }
+    // ---- store: last stage outputs e = b + r * (L / R_last) (b < L / R_last)
+    {
+        constexpr int RL = S::R(NST - 1), NB = E / RL, STR = L / RL;
+        const uint32_t oi = (NST == 1) ? li : si, ol = (NST == 1) ? ll : sl;
+        const uint64_t inst = first + oi;
+        if (inst < A.batch) {
+            const uint64_t hi = inst / A.out_C, lo = inst - hi * A.out_C;
+            const int64_t base = (int64_t)hi * A.out_hi + (int64_t)lo * A.out_lo;
+            cx<T>* dst = (cx<T>*)A.out;
+            const T sc = (T)A.scale;
+#pragma unroll
+            for (int i = 0; i < NB; ++i)
+#pragma unroll
+                for (int r = 0; r < RL; ++r) {
+                    const int e = (int)ol + i * TPI + r * STR;
+                    cx<T> y = v[i * RL + r];
+                    if (A.flags & KFA_FAST_TW4)
+                        y = cmul(y, ((const cx<T>*)A.tw4)[(uint32_t)e * A.out_C + (uint32_t)lo]);
+                    if (A.flags & KFA_FAST_SCALE) {
+                        y.re *= sc;
+                        y.im *= sc;
+                    }
+                    if (A.flags & KFA_FAST_CONJ_OUT)
+                        y.im = -y.im;
+                    cx<T>* p = dst + base + (int64_t)e * A.out_stride;
+                    if (A.flags & KFA_FAST_NT_OUT)
+                        store_nt(p, y);
+                    else
+                        *p = y;
+                }
+        }
+    }
+}
+
+template <int L, int MODE, typename T, int MUL>
+constexpr int fast_ipw() {
+    // instances per workgroup: ~256 threads for row passes; column passes take
+    // enough instances for >= 128-byte runs (times MUL), capped by threads and LDS
+    constexpr int TPI = P2<L>::TPI;
+    constexpr int PSH = sizeof(T) == 8 ? KFA_PAD_SHIFT_F64 : KFA_PAD_SHIFT_F32;
+    constexpr int bytes = (int)sizeof(cx<T>) * ((P2<L>::PAD_L(PSH)) | 1); // per instance
+    constexpr int thr = (1024 / TPI) > 0 ? (1024 / TPI) : 1;
+    constexpr int cap = (MUL == 1 ? 65536 : 160 * 1024) / bytes > 0
+                            ? (MUL == 1 ? 65536 : 160 * 1024) / bytes : 1;
+    if (MODE == FAST_ROW) {
+        constexpr int want = 256 / TPI > 0 ? 256 / TPI : 1;
+        constexpr int c1 = (65536 / bytes) > 0 ? (65536 / bytes) : 1;
+        return want < c1 ? want : c1;
+    }
+    constexpr int want = (sizeof(T) == 8 ? 8 : 16) * MUL;
+    constexpr int w = want < cap ? want : cap;
+    return w < thr ? w : thr;
+}
+
+template <typename T, int L, int MODE, int MUL>
+static hipError_t launch_fast_t(const FastArgs& a, hipStream_t s) {
+    constexpr int IPW = fast_ipw<L, MODE, T, MUL>();
+    constexpr int PSH = sizeof(T) == 8 ? KFA_PAD_SHIFT_F64 : KFA_PAD_SHIFT_F32;
+    constexpr int SP = (P2<L>::PAD_L(PSH)) | 1;
+    const uint32_t lds = P2<L>::S > 1 ? (uint32_t)(IPW * SP * sizeof(cx<T>)) : 0u;
+    static_assert(IPW * SP * sizeof(cx<T>) <= 160 * 1024, "LDS budget");
+    static_assert(IPW * P2<L>::TPI <= 1024, "workgroup size");
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_fast<T, L, MODE, IPW>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess)
+            return e;
+        attr = true;
+    }
+    const uint64_t groups = (a.batch + IPW - 1) / IPW;
+    if (!groups)
+        return hipSuccess;
+    hipLaunchKernelGGL((k_fast<T, L, MODE, IPW>), dim3((unsigned)groups),
+                       dim3(IPW * P2<L>::TPI), lds, s, a);
+    return hipGetLastError();
+}
+
+// column-pass width multiplier (1, 2 or 4; default 2, measured), fixed per process
+// (KFFT_AMD_COLS_MUL)
+static inline int fast_cols_mul() {
+    static const int m = [] {
+        const char* e = getenv("KFFT_AMD_COLS_MUL");
+        const int v = e ? atoi(e) : 2;
+        return (v == 2 || v == 4) ? v : 1;
+    }();
+    return m;
+}
+
+template <typename T, int L, int MODE>
+static hipError_t launch_fast_sel(const FastArgs& a, hipStream_t s) {
+    if constexpr (MODE != FAST_ROW && L >= 128 && L <= 1024) {
+        const int m = fast_cols_mul();
+        if (m == 2)
+            return launch_fast_t<T, L, MODE, 2>(a, s);
+        if (m == 4)
+            return launch_fast_t<T, L, MODE, 4>(a, s);
+    }
+    return launch_fast_t<T, L, MODE, 1>(a, s);
+}
+
+template <typename T, int MODE>
+static hipError_t launch_fast_mode(uint32_t L, const FastArgs& a, hipStream_t s) {
+    switch (L) {
+    case 16: return launch_fast_sel<T, 16, MODE>(a, s);
+    case 32: return launch_fast_sel<T, 32, MODE>(a, s);
+    case 64: return launch_fast_sel<T, 64, MODE>(a, s);
+    case 128: return launch_fast_sel<T, 128, MODE>(a, s);
+    case 256: return launch_fast_sel<T, 256, MODE>(a, s);
+    case 512: return launch_fast_sel<T, 512, MODE>(a, s);
+    case 1024: return launch_fast_sel<T, 1024, MODE>(a, s);
+    case 2048: return launch_fast_sel<T, 2048, MODE>(a, s);
+    case 4096: return launch_fast_sel<T, 4096, MODE>(a, s);
+    default: return hipErrorNotSupported;
+    }
+}
+
+template <typename T>
+static hipError_t launch_fast_any(uint32_t L, int mode, const FastArgs& a, hipStream_t s) {
+    if (mode == FAST_COL)
+        return launch_fast_mode<T, FAST_COL>(L, a, s);
+    if (mode == FAST_TOUT)
+        return launch_fast_mode<T, FAST_TOUT>(L, a, s);
+    return launch_fast_mode<T, FAST_ROW>(L, a, s);
+}
+
+} // namespace kfa

@@ -10,9 +10,15 @@ namespace kfa {
 hipError_t launch_fft(int prec, int maxe, const KfaFftArgs& a, uint32_t lds_bytes, hipStream_t s);
 hipError_t launch_fft_f64(int maxe, const KfaFftArgs& a, uint32_t lds_bytes, hipStream_t s);
 hipError_t launch_fft_f32(int maxe, const KfaFftArgs& a, uint32_t lds_bytes, hipStream_t s);
+// compile-time shaped power-of-two passes; mode: 0 row, 1 column, 2 row-in/column-out
+hipError_t launch_fast_f64(uint32_t L, int mode, const FastArgs& a, hipStream_t s);
+hipError_t launch_fast_f32(uint32_t L, int mode, const FastArgs& a, hipStream_t s);
+inline bool fast_len(uint32_t L) { return L >= 16 && L <= 4096 && (L & (L - 1)) == 0; }
 hipError_t launch_stage_twiddles(int prec, void* tw, const KfaFftArgs& a, hipStream_t s);
 hipError_t launch_twiddle_table(void* out_d, uint64_t count, uint64_t step, uint64_t M,
                                 hipStream_t s);
+hipError_t launch_twiddle_2d(void* out_d, uint64_t rows, uint64_t cols, uint64_t M,
+                             hipStream_t s);
 hipError_t launch_twiddle_indexed(void* out_d, const uint32_t* idx, uint64_t count, uint64_t M,
                                   int conjugate, hipStream_t s);
 hipError_t launch_bluestein_tables(void* chirp_d, void* h_d, uint64_t N, uint64_t M,

@@ -1,0 +1,167 @@
+"""GPU parity of the real-input, 2D and convolution entry points against the
+CPU oracle, including the reference's scaling quirks (even-length real inverse
+gives 2x, DISABLE_NORM gives N x) and output layouts (N/2+1 bins, EXPAND).
+
+Bar: relative L2 <= 1e-12 * log2(size) in the double build.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import kfft_amd as K
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))
+
+
+def tol(n):
+    return 1e-12 * max(1.0, math.log2(n))
+
+
+def cplx(rng, *shape):
+    return rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)
+
+
+REAL_N = [2, 3, 4, 8, 9, 15, 30, 31, 64, 100, 1000, 1009, 2018, 2080, 4096, 4160, 8192, 65536,
+          1 << 17, 1 << 18]
+
+
+@pytest.mark.parametrize("n", REAL_N)
+@pytest.mark.parametrize("expand", [False, True])
+def test_real_forward(n, expand):
+    rng = np.random.default_rng(n + 11)
+    batch = 3 if n <= 70000 else 1
+    x = rng.uniform(-1, 1, (batch, n))
+    flags = K.FLAG_EXPAND_SCALAR if expand else 0
+    got = K.ScalarPlan(n, flags).forward(x)
+    ref = O.ScalarPlan(n, flags).forward(x)
+    assert got.shape == ref.shape
+    assert rel(got, ref) <= tol(n)
+
+
+@pytest.mark.parametrize("n", REAL_N)
+@pytest.mark.parametrize("norm", [True, False])
+def test_real_inverse(n, norm):
+    rng = np.random.default_rng(n + 12)
+    batch = 3 if n <= 70000 else 1
+    x = rng.uniform(-1, 1, (batch, n))
+    X = O.ScalarPlan(n).forward(x)
+    flags = K.FLAG_INVERSE | (0 if norm else K.FLAG_DISABLE_NORM)
+    got = K.ScalarPlan(n, flags).inverse(X)
+    ref = O.ScalarPlan(n, flags).inverse(X)
+    assert rel(got, ref) <= tol(n)
+    k = (2.0 if n % 2 == 0 else 1.0) if norm else float(n)
+    assert rel(got, k * x) <= tol(n)
+
+
+def test_real_improper_plan():
+    x = np.zeros(16)
+    with pytest.raises(K.KfftError, match="improper"):
+        K.ScalarPlan(16, K.FLAG_INVERSE).forward(x)
+    with pytest.raises(K.KfftError, match="improper"):
+        K.ScalarPlan(16).inverse(np.zeros(9, np.complex128))
+
+
+def test_real_device_batch():
+    import torch
+
+    n, batch = 4096, 64
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-1, 1, (batch, n))
+    ref = O.ScalarPlan(n).forward(x)
+    xd = torch.from_numpy(x).cuda()
+    got = K.ScalarPlan(n).forward(xd).cpu().numpy()
+    assert rel(got, ref) <= tol(n)
+
+
+SHAPES_2D = [(6, 10), (7, 5), (16, 12), (11, 13), (64, 48), (128, 256), (1009, 8), (8, 2018),
+             (96, 4160)]
+
+
+@pytest.mark.parametrize("y,x", SHAPES_2D)
+def test_2d_complex(y, x):
+    rng = np.random.default_rng(x * y)
+    a = cplx(rng, y, x)
+    A = K.Cpx2Plan(x, y)(a)
+    assert rel(A, O.fft2(a)) <= tol(x * y)
+    back = K.Cpx2Plan(x, y, K.FLAG_INVERSE)(A)
+    assert rel(back, O.fft2(A, O.FLAG_INVERSE)) <= tol(x * y)
+
+
+@pytest.mark.parametrize("y,x", SHAPES_2D)
+def test_2d_scalar(y, x):
+    rng = np.random.default_rng(x + 7 * y)
+    a = rng.uniform(-1, 1, (y, x))
+    A = K.Scalar2Plan(x, y).forward(a)
+    assert rel(A, O.rfft2(a)) <= tol(x * y)
+    back = K.Scalar2Plan(x, y, K.FLAG_INVERSE).inverse(A)
+    assert rel(back, O.irfft2(A)) <= tol(x * y)
+    raw = K.Scalar2Plan(x, y, K.FLAG_INVERSE | K.FLAG_DISABLE_NORM).inverse(A)
+    assert rel(raw, O.irfft2(A, O.FLAG_INVERSE | O.FLAG_DISABLE_NORM)) <= tol(x * y)
+
+
+@pytest.mark.parametrize("n", [12, 13, 30, 1000, 1009, 4096])
+def test_conv_1d(n):
+    rng = np.random.default_rng(n)
+    a, b = cplx(rng, n), cplx(rng, n)
+    assert rel(K.ConvPlan(n)(a, b), O.conv(a, b)) <= tol(n)
+    ar, br = a.real.copy(), b.real.copy()
+    assert rel(K.ConvPlan(n, real=True)(ar, br), O.conv_scalar(ar, br)) <= tol(n)
+
+
+@pytest.mark.parametrize("y,x", [(8, 8), (6, 10), (7, 6), (64, 64), (60, 1009)])
+def test_conv_2d(y, x):
+    rng = np.random.default_rng(x * y + 5)
+    a, b = rng.uniform(-1, 1, (y, x)), rng.uniform(-1, 1, (y, x))
+    assert rel(K.Conv2Plan(x, y, real=True)(a, b), O.conv2_scalar(a, b)) <= tol(x * y)
+    ac, bc = cplx(rng, y, x), cplx(rng, y, x)
+    got = K.Conv2Plan(x, y)(ac, bc)
+    if x == y:
+        assert rel(got, O.conv2(ac, bc)) <= tol(x * y)
+    # every shape: the true 2D circular convolution (the reference is defined only
+    # for square inputs, ext/conv2d/kfft_cpx_conv.c:67)
+    ref = np.fft.ifft2(np.fft.fft2(ac) * np.fft.fft2(bc))
+    assert rel(got, ref) <= tol(x * y)
+
+
+def test_conv2d_padded_image():
+    """A reduced instance of the real 2D convolution workload: a 256x256 image
+    convolved with a 15x15 kernel, both zero-padded to 272x272, against a direct
+    convolution on a crop, times the reference's x2 (even y)."""
+    rng = np.random.default_rng(42)
+    img = rng.uniform(0, 1, (256, 256))
+    yy, xx = np.mgrid[-7:8, -7:8]
+    ker = np.exp(-(xx ** 2 + yy ** 2) / 18.0) + 0.01 * rng.uniform(-1, 1, (15, 15))
+    P = 272
+    a = np.zeros((P, P))
+    a[:256, :256] = img
+    b = np.zeros((P, P))
+    b[:15, :15] = ker
+    got = K.Conv2Plan(P, P, real=True)(a, b)
+    crop = slice(20, 84)
+    direct = np.zeros((64, 64))
+    for dy in range(15):
+        for dx in range(15):
+            direct += ker[dy, dx] * a[20 - dy:84 - dy, 20 - dx:84 - dx]
+    assert rel(got[crop, crop], 2.0 * direct) <= 1e-11
+
+
+def test_hadamard_transpose_device():
+    import torch
+    import ctypes as C
+
+    lib = K.load(False)
+    rng = np.random.default_rng(2)
+    a = cplx(rng, 12, 20)
+    b = cplx(rng, 12, 20)
+    ad, bd = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    lib.kfft_math_hadamard_cpx(C.c_void_p(ad.data_ptr()), C.c_void_p(bd.data_ptr()), a.size)
+    assert rel(ad.cpu().numpy(), a * b) <= 1e-15
+    od = torch.empty(20 * 12, dtype=torch.complex128, device="cuda")
+    lib.kfft_math_transpose_cpx(C.c_void_p(bd.data_ptr()), C.c_void_p(od.data_ptr()), 20, 12)
+    assert np.array_equal(od.cpu().numpy().reshape(20, 12), b.T)
