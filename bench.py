@@ -57,6 +57,18 @@ def _gpu_form(form: str, n: int, batch: int, steps: int, warmup: int, x64):
     return plan, y, ms, wall
 
 
+def _measured_traffic(form: str, n: int, batch: int):
+    """HBM bytes of one step from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary for this workload (profiles/traffic_*.json), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_c2c_{n}x{batch}.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d[form]["bytes_per_step"]
+    except Exception:
+        return None
+
+
 def cpu_baseline(n: int, sample_batch: int):
     import numpy as np
 
@@ -94,6 +106,9 @@ def main():
     ap.add_argument("--form", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--cpu-sample", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-ref", action="store_true",
+                    help="after timing, run one plain device copy of the FP64 input (4 GiB at the "
+                         "default size) so that profiler byte counters can be calibrated")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the batch shards trivially across "
@@ -154,6 +169,15 @@ def main():
         "fp32_rel_l2_vs_fp64": err32,
         "fp32_speedup_vs_fp64": round(res["fp64"][2] / res["fp32"][2], 3),
     }
+    if args.copy_ref:
+        ref = torch.empty_like(x64)
+        ref.copy_(x64)
+        torch.cuda.synchronize()
+        out["copy_ref_bytes"] = 2 * x64.numel() * x64.element_size()
+        del ref
+    traffic = _measured_traffic(form, n, batch)
+    if traffic is not None:
+        out["roofline"]["traffic"] = traffic
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_sample)
     print(json.dumps(out), flush=True)

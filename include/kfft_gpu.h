@@ -19,6 +19,9 @@ KFFT_API kfft_return_t kfft_gpu_memcpy_d2h(void* dst, const void* src, size_t by
 KFFT_API kfft_return_t kfft_gpu_synchronize(void);
 KFFT_API void* kfft_gpu_stream(void);
 KFFT_API const char* kfft_gpu_last_error(void);
+/* non-zero when an asynchronous launch of this complex plan reported a failed
+ * dependency wait since the last call (synchronises the library stream) */
+KFFT_API int kfft_gpu_plan_error(const kfft_plan_cpx* plan);
 /* one line describing how a complex, scalar or 2D plan is executed */
 KFFT_API const char* kfft_gpu_plan_describe(const void* plan);
 

@@ -46,6 +46,9 @@ enum kfft_eval_flags {
     KFFT_FLAG_GPU_SPLIT_F16 = 1 << 8,
     KFFT_FLAG_GPU_SPLIT_BF16 = 1 << 9,
     KFFT_FLAG_GPU_REFINE = 1 << 10,
+    /* GPU: run a 256x256 two-pass transform as one persistent launch that keeps the
+     * intermediate in each XCD's L2 (experimental; library stream only) */
+    KFFT_FLAG_GPU_ONCHIP = 1 << 11,
 };
 
 /* incs/kfft_types.h:65 */

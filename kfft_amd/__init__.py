@@ -29,6 +29,7 @@ FLAG_DISABLE_NORM = 1 << 5
 FLAG_GPU_SPLIT_F16 = 1 << 8
 FLAG_GPU_SPLIT_BF16 = 1 << 9
 FLAG_GPU_REFINE = 1 << 10
+FLAG_GPU_ONCHIP = 1 << 11
 
 RET_SUCCESS, RET_ALLOC_FAIL, RET_BUFFER_FAIL, RET_FREE_NULL = 0, 1, 2, 3
 RET_IMPROPER_PLAN, RET_BADARGUMENTS, RET_GPU_FAIL = 4, 5, 6
@@ -52,6 +53,7 @@ EXPORTS = [
     "kfft_config2_conv_scalar", "kfft_eval2_conv_scalar",
     "kfft_gpu_malloc", "kfft_gpu_free", "kfft_gpu_memcpy_h2d", "kfft_gpu_memcpy_d2h",
     "kfft_gpu_synchronize", "kfft_gpu_stream", "kfft_gpu_last_error", "kfft_gpu_plan_describe",
+    "kfft_gpu_plan_error",
     "kfft_eval_cpx_batch", "kfft_eval_cpx_batch_async",
     "kfft_scalar_out_len", "kfft_eval_scalar_batch", "kfft_evali_scalar_batch",
     "kfft_eval_scalar_batch_async", "kfft_evali_scalar_batch_async",
@@ -126,6 +128,7 @@ def load(half: bool = False):
         "kfft_gpu_stream": (vp, []),
         "kfft_gpu_last_error": (C.c_char_p, []),
         "kfft_gpu_plan_describe": (C.c_char_p, [vp]),
+        "kfft_gpu_plan_error": (C.c_int, [vp]),
         "kfft_eval_cpx_batch": (ret, [vp, vp, vp, u32]),
         "kfft_eval_cpx_batch_async": (ret, [vp, vp, vp, u32, vp]),
         "kfft_scalar_out_len": (u32, [vp]),
@@ -238,6 +241,10 @@ class CpxPlan(_Plan):
             check(self.L.kfft_eval_cpx_batch(self._p, _ptr(x), _ptr(out), batch),
                   "kfft_eval_cpx_batch", self.half)
         return out
+
+    def error(self) -> int:
+        """Non-zero if an asynchronous launch reported a failed dependency wait."""
+        return self.L.kfft_gpu_plan_error(self._p)
 
     def run_async(self, x, out, batch: int, stream=None):
         check(self.L.kfft_eval_cpx_batch_async(self._p, _ptr(x), _ptr(out), batch, stream),

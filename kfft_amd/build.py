@@ -29,7 +29,7 @@ CFLAGS = [
 ]
 
 ENGINE = ["kfa_fft_f64.hip", "kfa_fft_f32.hip", "kfa_fast_f64.hip", "kfa_fast_f32.hip",
-          "kfa_aux.hip", "kfa_engine.cpp"]
+          "kfa_fused.hip", "kfa_aux.hip", "kfa_engine.cpp"]
 VARIANTS = {
     "libkfft_amd.so": [],
     "libkfft_amd_half.so": ["-DKFFT_HALF_SCALAR"],

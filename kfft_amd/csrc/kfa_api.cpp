@@ -297,9 +297,11 @@ static kfft_return_t cpx_run_dev(kfft_plan_cpx* P, const void* din, void* dout, 
     const Plan1D& pl = *g->p;
     if (!g->scratch.ensure(pl.scratch_bytes(batch)))
         return gpu_fail("scratch");
+    Fuse fz;
+    fz.onchip = (P->flags & KFFT_FLAG_GPU_ONCHIP) != 0;
     hipError_t e =
         pl.execute(contiguous(din, P->nfft), contiguous(dout, P->nfft), batch,
-                   (P->flags & KFFT_FLAG_INVERSE) != 0, cpx_scale(P), g->scratch.p, s);
+                   (P->flags & KFFT_FLAG_INVERSE) != 0, cpx_scale(P), g->scratch.p, s, &fz);
     return hip_ok(e, "cpx execute") ? KFFT_RET_SUCCESS : KFFT_RET_GPU_FAIL;
 }
 
@@ -321,6 +323,10 @@ static kfft_return_t cpx_run(kfft_plan_cpx* P, const kfft_cpx* fin, kfft_cpx* fo
         return r;
     if (!finish_out(so, s) || !hip_ok(hipStreamSynchronize(s), "sync"))
         return KFFT_RET_GPU_FAIL;
+    if (g->p->async_error()) {
+        set_error("fused transform: a dependency wait timed out");
+        return KFFT_RET_GPU_FAIL;
+    }
     return KFFT_RET_SUCCESS;
 }
 
@@ -1275,6 +1281,13 @@ KFFT_API kfft_return_t kfft_gpu_synchronize(void) {
                                                                  : KFFT_RET_GPU_FAIL;
 }
 KFFT_API void* kfft_gpu_stream(void) { return runtime_init() ? (void*)default_stream() : nullptr; }
+
+KFFT_API int kfft_gpu_plan_error(const kfft_plan_cpx* plan) {
+    if (!plan || !plan->gpu)
+        return 0;
+    CpxObj* g = dynamic_cast<CpxObj*>((GpuObj*)plan->gpu);
+    return (g && g->p) ? g->p->async_error() : 0;
+}
 KFFT_API const char* kfft_gpu_last_error(void) { return last_error(); }
 
 KFFT_API const char* kfft_gpu_plan_describe(const void* plan) {

@@ -92,3 +92,20 @@ struct FastArgs {
     double scale;
 };
 
+
+// Arguments of the fused on-chip two-pass transform (kfa_fused_impl.h).
+struct FusedArgs {
+    const void* in;
+    void* out;
+    int64_t in_dist, out_dist; // elements between transforms
+    void* ring;                // nq * S slots of N1*N2 elements
+    const void* tw_col;        // stage twiddles of the N1 pass
+    const void* tw_row;        // stage twiddles of the N2 pass
+    const void* tw4;           // [k1][n2] four-step twiddles
+    uint32_t* ctr;             // tickets (16 words apart), doneA[T], doneB[T], error
+    uint64_t T;                // transforms in this launch
+    uint32_t S, nq;
+    uint32_t flags; // KFA_FAST_CONJ_IN / CONJ_OUT / SCALE
+    uint32_t spin_limit;
+    double scale;
+};

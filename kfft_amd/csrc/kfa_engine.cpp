@@ -404,8 +404,104 @@ hipError_t SubFft::run_four_fast(const KfaIo& load, const KfaIo& store, double s
     return launch_fast(prec, N2, 2, b, s);
 }
 
+// ============================================================ fused two-pass
+static uint32_t env_u32(const char* name, uint32_t dflt) {
+    const char* e = getenv(name);
+    return e ? (uint32_t)atoi(e) : dflt;
+}
+
+static uint32_t g_num_cus = 0;
+
+uint32_t fused_queues() {
+    static std::once_flag once;
+    static uint32_t nq = 0;
+    std::call_once(once, [] {
+        if (env_u32("KFFT_AMD_FUSED", 1) == 0 || !runtime_init())
+            return;
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus <= 0)
+            return;
+        g_num_cus = (uint32_t)cus;
+        const uint32_t groups = 2 * (uint32_t)cus;
+        DevBuf ids;
+        if (!ids.alloc(groups * 4))
+            return;
+        hipStream_t s = default_stream();
+        if (launch_xcc_probe((uint32_t*)ids.p, groups, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return;
+        std::vector<uint32_t> h(groups);
+        if (hipMemcpy(h.data(), ids.p, groups * 4, hipMemcpyDeviceToHost) != hipSuccess)
+            return;
+        uint32_t seen = 0, maxid = 0;
+        for (uint32_t v : h) {
+            maxid = std::max(maxid, v);
+            if (v < 32)
+                seen |= 1u << v;
+        }
+        // every XCD from 0 to maxid must host workgroups of an ordinary grid
+        if (maxid < 16 && seen == ((1u << (maxid + 1)) - 1) && maxid >= 1)
+            nq = maxid + 1;
+    });
+    return nq;
+}
+
+hipError_t SubFft::run_fused(const KfaIo& load, const KfaIo& store, double scale, uint64_t batch,
+                             hipStream_t s) const {
+    std::lock_guard<std::mutex> lk(fused_mu);
+    const uint32_t nq = fused_queues();
+    const uint32_t S = env_u32("KFFT_AMD_FUSED_SLOTS", prec == 8 ? 4 : 6);
+    const uint32_t wpc = env_u32("KFFT_AMD_FUSED_WGS", 2);
+    const size_t words = 16 * (size_t)nq + 2 * batch + 1;
+    if (!fused_ctr.ensure(words * 4) ||
+        !fused_ring.ensure((size_t)nq * S * L * elem_bytes(prec)))
+        return hipErrorOutOfMemory;
+    hipError_t e = launch_fill_u32((uint32_t*)fused_ctr.p, words, 0u, s);
+    if (e != hipSuccess)
+        return e;
+    FusedArgs f;
+    memset(&f, 0, sizeof(f));
+    f.in = (const void*)load.ptr;
+    f.out = (void*)store.ptr;
+    f.in_dist = load.dist_hi;
+    f.out_dist = store.dist_hi;
+    f.ring = fused_ring.p;
+    f.tw_col = col.tw.p;
+    f.tw_row = row.tw.p;
+    f.tw4 = tw4_full.p;
+    f.ctr = (uint32_t*)fused_ctr.p;
+    f.T = batch;
+    f.S = S;
+    f.nq = nq;
+    f.flags = ((load.flags & KFA_IO_CONJ1) ? KFA_FAST_CONJ_IN : 0) |
+              ((store.flags & KFA_IO_CONJ2) ? KFA_FAST_CONJ_OUT : 0) |
+              ((store.flags & KFA_IO_SCALE) ? KFA_FAST_SCALE : 0);
+    f.spin_limit = 1u << 22;
+    f.scale = scale;
+    fused_T = batch;
+    fused_used = true;
+    const uint32_t groups = std::max<uint32_t>(nq, wpc * g_num_cus);
+    return prec == 8 ? launch_fused_f64(N1, f, groups, s) : launch_fused_f32(N1, f, groups, s);
+}
+
+int SubFft::fused_error() const {
+    std::lock_guard<std::mutex> lk(fused_mu);
+    if (!fused_used || !fused_ctr.p)
+        return 0;
+    hipStream_t s = default_stream();
+    uint32_t v = 0;
+    const size_t off = (16 * (size_t)fused_queues() + 2 * fused_T) * 4;
+    if (hipStreamSynchronize(s) != hipSuccess ||
+        hipMemcpy(&v, (char*)fused_ctr.p + off, 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return -1;
+    fused_used = false;
+    return (int)v;
+}
+
 hipError_t SubFft::run(const KfaIo& load, const KfaIo& store, double scale, double scale_post,
-                       uint64_t batch, void* scratch, hipStream_t s) const {
+                       uint64_t batch, void* scratch, hipStream_t s, bool onchip) const {
     const bool simple = fast_io_ok(load, KFA_IO_CONJ1) &&
                         fast_io_ok(store, KFA_IO_CONJ2 | KFA_IO_SCALE) && scale_post == 1.0;
     if (!four) {
@@ -435,6 +531,9 @@ hipError_t SubFft::run(const KfaIo& load, const KfaIo& store, double scale, doub
     }
     if (load.C != 1 || store.C != 1)
         return hipErrorInvalidValue;
+    if (onchip && simple && tw4_full.p && N1 == 256 && N2 == 256 && s == default_stream() &&
+        load.stride == 1 && store.stride == 1 && fused_queues() > 0)
+        return run_fused(load, store, scale, batch, s);
     if (simple && tw4_full.p) {
         const Tuning& tn = tuning();
         uint64_t chunk = batch;
@@ -713,7 +812,7 @@ hipError_t Plan1D::execute(const Layout& in, const Layout& out, uint64_t batch, 
             store.flags |= KFA_IO_CONJ2;
         if (scaled)
             store.flags |= KFA_IO_SCALE;
-        return sub_.run(load, store, scale, 1.0, batch, scratch, s);
+        return sub_.run(load, store, scale, 1.0, batch, scratch, s, fuse && fuse->onchip);
     }
     if (fuse && fuse->pw_in)
         return hipErrorInvalidValue; // callers apply the product separately for these kinds

@@ -62,6 +62,7 @@ struct Layout {
 struct Fuse {
     const void* pw_in = nullptr; // pointwise multiply on load, same layout as the input
     bool pw_in_f64 = false;
+    bool onchip = false;         // allow the fused on-chip two-pass kernel
 };
 
 // -------------------------------------------------------------- LDS pass plans
@@ -100,10 +101,22 @@ struct SubFft {
     // Run with fully described first-load and last-store descriptors.  For
     // four-step the load/store must have C == 1 (contiguous instances).
     hipError_t run(const KfaIo& load, const KfaIo& store, double scale, double scale_post,
-                   uint64_t batch, void* scratch, hipStream_t s) const;
+                   uint64_t batch, void* scratch, hipStream_t s, bool onchip = false) const;
     hipError_t run_four_fast(const KfaIo& load, const KfaIo& store, double scale, uint64_t batch,
                              void* scratch, uint32_t nt, hipStream_t s) const;
+    // fused on-chip variant of the two passes (N1 == N2 == 256, library stream only)
+    hipError_t run_fused(const KfaIo& load, const KfaIo& store, double scale, uint64_t batch,
+                         hipStream_t s) const;
+    // non-zero if a fused launch since the last check timed out waiting (syncs the stream)
+    int fused_error() const;
+    mutable std::mutex fused_mu;
+    mutable DevBuf fused_ctr, fused_ring;
+    mutable uint64_t fused_T = 0;
+    mutable bool fused_used = false;
 };
+
+// queues of the fused path (the number of XCDs), 0 when it is unavailable
+uint32_t fused_queues();
 
 bool subfft_capable(uint32_t L, int prec);
 
@@ -126,6 +139,7 @@ class Plan1D {
     hipError_t execute(const Layout& in, const Layout& out, uint64_t batch, bool inverse,
                        double scale, void* scratch, hipStream_t s, const Fuse* fuse = nullptr) const;
     bool supports_columns() const;
+    int async_error() const { return sub_.fused_error(); }
 
   private:
     uint32_t n_;

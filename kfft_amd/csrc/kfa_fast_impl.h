@@ -85,17 +85,15 @@ __device__ __forceinline__ void store_nt(cx<T>* p, cx<T> x) {
     __builtin_nontemporal_store(v, reinterpret_cast<V*>(p));
 }
 
+// One workgroup tile: instances [first, first + IPW) of the pass described by A.
+// The caller provides the LDS area (IPW * SP elements).
 template <typename T, int L, int MODE, int IPW>
-__global__ void __launch_bounds__(IPW* P2<L>::TPI) k_fast(const FastArgs A) {
+__device__ __forceinline__ void fast_tile(const FastArgs& A, uint64_t first, cx<T>* lds) {
     using S = P2<L>;
     constexpr int E = S::E, TPI = S::TPI, NST = S::S;
     constexpr int PSH = sizeof(T) == 8 ? KFA_PAD_SHIFT_F64 : KFA_PAD_SHIFT_F32;
     constexpr int SP = (S::PAD_L(PSH)) | 1; // odd LDS stride per instance
-    extern __shared__ __align__(16) unsigned char kfa_smem[];
-    cx<T>* lds = reinterpret_cast<cx<T>*>(kfa_smem);
-
     const uint32_t tid = threadIdx.x;
-    const uint64_t first = (uint64_t)blockIdx.x * IPW;
     // load side mapping
     const uint32_t li = (MODE == FAST_COL) ? tid % IPW : tid / TPI;
     const uint32_t ll = (MODE == FAST_COL) ? tid / IPW : tid % TPI;
@@ -231,6 +229,12 @@ __global__ void __launch_bounds__(IPW* P2<L>::TPI) k_fast(const FastArgs A) {
     }
 }
 
+template <typename T, int L, int MODE, int IPW>
+__global__ void __launch_bounds__(IPW* P2<L>::TPI) k_fast(const FastArgs A) {
+    extern __shared__ __align__(16) unsigned char kfa_smem[];
+    fast_tile<T, L, MODE, IPW>(A, (uint64_t)blockIdx.x * IPW, reinterpret_cast<cx<T>*>(kfa_smem));
+}
+
 template <int L, int MODE, typename T, int MUL>
 constexpr int fast_ipw() {
     // instances per workgroup: ~256 threads for row passes; column passes take
@@ -263,8 +267,10 @@ static hipError_t launch_fast_t(const FastArgs& a, hipStream_t s) {
     if (!attr) {
         hipError_t e = hipFuncSetAttribute((const void*)k_fast<T, L, MODE, IPW>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        if (e != hipSuccess)
+        if (e != hipSuccess) {
+            (void)hipGetLastError(); // do not leave the error for the next launch
             return e;
+        }
         attr = true;
     }
     const uint64_t groups = (a.batch + IPW - 1) / IPW;

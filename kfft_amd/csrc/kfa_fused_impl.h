@@ -1,0 +1,177 @@
+// Fused two-pass transform with the intermediate kept in each XCD's L2.
+//
+// A persistent grid works through per-XCD ticket queues.  Transform t belongs
+// to queue t mod nq; the XCD a workgroup runs on (hardware register XCC_ID)
+// selects its queue, so both passes of a transform run on one XCD and the
+// intermediate never leaves that XCD's L2.  Per transform there are NA
+// column tiles (pass A: length-N1 FFTs down the columns, W_N^(n2 k1) on the
+// way out) and NB row tiles (pass B: length-N2 FFTs along the rows, natural
+// order store).  Tickets of a queue run block by block; block k holds the A
+// tiles of local transform k followed by the B tiles of local transform k-1,
+// so B tiles are usually ready when they are taken.
+//
+// The intermediate lives in a ring of S slots per queue.  Ordering:
+//   A(k) waits until B(k-S) has finished reading slot k mod S;
+//   B(k) waits until all NA tiles of A(k) have stored their results.
+// A tile signals completion after its stores are acknowledged by L2
+// (s_waitcnt vmcnt(0) + barrier, then a device-scope atomic add); a waiter
+// polls with device-scope atomic RMWs, then invalidates its CU's L1
+// (buffer_inv sc0) before reading the slot.  Each wait targets tiles that
+// were dequeued earlier by running workgroups, so it always ends; a bounded
+// spin still raises an error flag instead of hanging.
+#pragma once
+#include "kfa_fast_impl.h"
+
+namespace kfa {
+
+
+__device__ __forceinline__ uint32_t xcc_id() {
+    return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u; // HW_REG_XCC_ID[3:0]
+}
+
+// thread 0 only: wait until *p >= target (bounded); returns false on timeout
+__device__ __forceinline__ bool wait_geq(uint32_t* p, uint32_t target, uint32_t limit,
+                                         uint32_t* err) {
+    for (uint32_t it = 0;; ++it) {
+        const uint32_t v = __hip_atomic_fetch_add(p, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (v >= target)
+            return true;
+        if (it >= limit) {
+            __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(4);
+    }
+}
+
+// all of this workgroup's global stores acknowledged, then one increment
+__device__ __forceinline__ void signal_done(uint32_t* p) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T, int N1, int N2, int IPW>
+__global__ void __launch_bounds__(IPW* P2<N1>::TPI) k_fused4(const FusedArgs F) {
+    static_assert(P2<N1>::TPI == P2<N2>::TPI, "both passes use the same thread count");
+    constexpr uint32_t L = (uint32_t)N1 * N2;
+    constexpr uint32_t NA = N2 / IPW, NB = N1 / IPW, PER = NA + NB, LAG = 1;
+    extern __shared__ __align__(16) unsigned char kfa_smem[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(kfa_smem);
+    __shared__ uint32_t sh_ticket;
+
+    const uint32_t q = xcc_id() % F.nq;
+    const uint64_t Tq = F.T > q ? (F.T - q + F.nq - 1) / F.nq : 0;
+    const uint64_t total = (Tq + LAG) * PER;
+    uint32_t* tick = F.ctr + 16 * q;
+    uint32_t* doneA = F.ctr + 16 * F.nq;
+    uint32_t* doneB = doneA + F.T;
+    uint32_t* err = doneB + F.T;
+    cx<T>* ring = (cx<T>*)F.ring + (size_t)q * F.S * L;
+
+    for (;;) {
+        __syncthreads(); // LDS and sh_ticket reuse
+        if (threadIdx.x == 0)
+            sh_ticket = __hip_atomic_fetch_add(tick, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const uint32_t tk = sh_ticket;
+        if (tk >= total)
+            break;
+        const uint64_t blk = tk / PER;
+        const uint32_t j = tk - (uint32_t)(blk * PER);
+        if (j < NA) {
+            if (blk >= Tq)
+                continue;
+            const uint64_t tl = blk, t = tl * F.nq + q;
+            if (tl >= F.S) {
+                if (threadIdx.x == 0)
+                    wait_geq(doneB + (tl - F.S) * F.nq + q, NB, F.spin_limit, err);
+                __syncthreads();
+            }
+            FastArgs a;
+            a.in = (const cx<T>*)F.in + (int64_t)t * F.in_dist;
+            a.out = ring + (size_t)(tl % F.S) * L;
+            a.tw = F.tw_col;
+            a.tw4 = F.tw4;
+            a.in_hi = 0;
+            a.in_lo = 1;
+            a.in_stride = N2;
+            a.out_hi = 0;
+            a.out_lo = 1;
+            a.out_stride = N2;
+            a.batch = N2;
+            a.in_C = N2;
+            a.out_C = N2;
+            a.flags = KFA_FAST_TW4 | KFA_FAST_NT_IN | (F.flags & KFA_FAST_CONJ_IN);
+            a.pad = 0;
+            a.scale = 1.0;
+            fast_tile<T, N1, FAST_COL, IPW>(a, (uint64_t)j * IPW, lds);
+            signal_done(doneA + t);
+        } else {
+            if (blk < LAG)
+                continue;
+            const uint64_t tl = blk - LAG, t = tl * F.nq + q;
+            if (tl >= Tq)
+                continue;
+            if (threadIdx.x == 0)
+                wait_geq(doneA + t, NA, F.spin_limit, err);
+            __syncthreads();
+            asm volatile("buffer_inv sc0" ::: "memory"); // drop this CU's stale L1 lines
+            FastArgs b;
+            b.in = ring + (size_t)(tl % F.S) * L;
+            b.out = (cx<T>*)F.out + (int64_t)t * F.out_dist;
+            b.tw = F.tw_row;
+            b.tw4 = nullptr;
+            b.in_hi = 0;
+            b.in_lo = N2;
+            b.in_stride = 1;
+            b.out_hi = 0;
+            b.out_lo = 1;
+            b.out_stride = N1;
+            b.batch = N1;
+            b.in_C = N1;
+            b.out_C = N1;
+            b.flags = KFA_FAST_NT_OUT | (F.flags & (KFA_FAST_CONJ_OUT | KFA_FAST_SCALE));
+            b.pad = 0;
+            b.scale = F.scale;
+            fast_tile<T, N2, FAST_TOUT, IPW>(b, (uint64_t)(j - NA) * IPW, lds);
+            signal_done(doneB + t);
+        }
+    }
+}
+
+// reads XCC_ID for every workgroup of a grid (plan-time check of the mapping)
+__global__ void k_xcc_probe(uint32_t* out) {
+    if (threadIdx.x == 0)
+        out[blockIdx.x] = xcc_id();
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
+template <typename T, int N, int IPW>
+static hipError_t launch_fused_t(const FusedArgs& f, uint32_t groups, hipStream_t s) {
+    constexpr int PSH = sizeof(T) == 8 ? KFA_PAD_SHIFT_F64 : KFA_PAD_SHIFT_F32;
+    constexpr int SP = (P2<N>::PAD_L(PSH)) | 1;
+    constexpr uint32_t lds = (uint32_t)(IPW * SP * sizeof(cx<T>));
+    static_assert(lds + 64 <= 160 * 1024, "LDS budget");
+    static bool attr = false;
+    if (!attr) {
+        // the kernel also has a few bytes of static LDS: request exactly what it needs
+        hipError_t e = hipFuncSetAttribute((const void*)k_fused4<T, N, N, IPW>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) {
+            (void)hipGetLastError(); // do not leave the error for the next launch
+            return e;
+        }
+        attr = true;
+    }
+    hipLaunchKernelGGL((k_fused4<T, N, N, IPW>), dim3(groups), dim3(IPW * P2<N>::TPI), lds, s, f);
+    return hipGetLastError();
+}
+
+} // namespace kfa

@@ -13,6 +13,11 @@ hipError_t launch_fft_f32(int maxe, const KfaFftArgs& a, uint32_t lds_bytes, hip
 // compile-time shaped power-of-two passes; mode: 0 row, 1 column, 2 row-in/column-out
 hipError_t launch_fast_f64(uint32_t L, int mode, const FastArgs& a, hipStream_t s);
 hipError_t launch_fast_f32(uint32_t L, int mode, const FastArgs& a, hipStream_t s);
+// fused on-chip two-pass transform of N*N points (N = 256)
+hipError_t launch_fused_f32(uint32_t N, const FusedArgs& f, uint32_t groups, hipStream_t s);
+hipError_t launch_fused_f64(uint32_t N, const FusedArgs& f, uint32_t groups, hipStream_t s);
+hipError_t launch_xcc_probe(uint32_t* out, uint32_t groups, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 inline bool fast_len(uint32_t L) { return L >= 16 && L <= 4096 && (L & (L - 1)) == 0; }
 hipError_t launch_stage_twiddles(int prec, void* tw, const KfaFftArgs& a, hipStream_t s);
 hipError_t launch_twiddle_table(void* out_d, uint64_t count, uint64_t step, uint64_t M,

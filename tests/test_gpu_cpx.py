@@ -174,3 +174,36 @@ def test_float_build_plain(n):
     X = K.CpxPlan(n, half=True)(x)
     ref = O.fft(x.astype(np.complex128))
     assert rel(X, ref) <= 2e-6 * max(1, math.log2(n)) / 4
+
+
+@pytest.mark.parametrize("half", [False, True])
+@pytest.mark.parametrize("batch", [1, 3, 8, 37, 300])
+def test_fused_two_pass_path(half, batch):
+    """2^16 runs as one persistent launch with the intermediate kept on chip
+    (per-XCD queues).  Edge cases: fewer transforms than XCDs, uneven queues.
+    Every instance is checked against the oracle (FP32 at the float bar)."""
+    import torch
+
+    n = 1 << 16
+    g = torch.Generator(device="cuda").manual_seed(batch)
+    x = torch.view_as_complex(torch.rand(batch, n, 2, generator=g, device="cuda",
+                                         dtype=torch.float64) * 2 - 1).contiguous()
+    xin = x.to(torch.complex64) if half else x
+    p = K.CpxPlan(n, K.FLAG_GPU_ONCHIP, half=half)
+    y = torch.empty_like(xin)
+    p.run_async(xin, y, batch)
+    K.synchronize(half)
+    assert p.error() == 0
+    X = y.cpu().numpy().astype(np.complex128)
+    xs = xin.cpu().numpy().astype(np.complex128)
+    bar = (5e-7 * math.log2(n)) if half else tol(n)
+    for b in sorted({0, batch // 2, batch - 1}):
+        assert rel(X[b], O.fft(xs[b])) <= bar, (b, p.describe())
+    inv = K.CpxPlan(n, K.FLAG_INVERSE | K.FLAG_GPU_ONCHIP, half=half)
+    z = torch.empty_like(y)
+    inv.run_async(y, z, batch)
+    K.synchronize(half)
+    assert inv.error() == 0
+    err = (torch.linalg.vector_norm((z - xin).to(torch.complex128)) /
+           torch.linalg.vector_norm(xin.to(torch.complex128))).item()
+    assert err <= bar
