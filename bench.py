@@ -6,10 +6,19 @@ timed region.  A step is one batched transform of the whole set through the
 C ABI (kfft_eval_cpx_batch_async on the library stream).  Timing uses HIP
 events recorded on that stream around exactly K steps.
 
-Forms: "fp32" runs the float build (libkfft_amd_half.so, FP32 arithmetic with
-FP64-generated twiddles); "fp64" runs the double build (the plain form).  The
-reported value is GFLOP/s = 5 N log2 N * batch / t.  Both forms are measured
-and their accuracy against the FP64 path is printed in the JSON line.
+Forms (all run in one invocation; --form picks the one reported as `value`):
+  fp64       the double build (libkfft_amd.so), the plain form
+  fp32       the float build (libkfft_amd_half.so): FP32 arithmetic on the
+             vector units, FP64-generated twiddles rounded once
+  f16, bf16  the precision form: radix-16 stages as split FP16 / BF16 matrix
+             products with FP32 accumulation (KFFT_FLAG_GPU_SPLIT_F16 / _BF16)
+  f16r, bf16r  the same plus one refinement step on the FP64 residual
+             (KFFT_FLAG_GPU_REFINE)
+The reported value is GFLOP/s = 5 N log2 N * batch / t.  Every FP32 form
+carries its relative L2 error against the FP64 path on two input sets (uniform
+noise, and unit noise with 1e-6-amplitude tones); the split forms carry their
+matrix-core utilisation (dense FP16/BF16 products issued per second against
+the 2.5 PFLOP/s peak).
 
 The CPU baseline is the oracle (a C restatement of the reference's algorithm,
 OpenMP over the batch, all host threads) timed on a bounded sample of the same
@@ -28,16 +37,50 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+MFMA_PEAK_FLOPS = 2.5e15  # MI355X dense FP16 / BF16 matrix peak
+FORMS = ("fp64", "fp32", "f16", "bf16", "f16r", "bf16r")
 
 
-def _gpu_form(form: str, n: int, batch: int, steps: int, warmup: int, x64):
+def _form_flags(form: str) -> int:
+    import kfft_amd as K
+
+    f = 0
+    if form.startswith("f16"):
+        f |= K.FLAG_GPU_SPLIT_F16
+    if form.startswith("bf16"):
+        f |= K.FLAG_GPU_SPLIT_BF16
+    if form.endswith("r"):
+        f |= K.FLAG_GPU_REFINE
+    return f
+
+
+def _radix16_stages(describe: str) -> int:
+    """Radix-16 stages of a power-of-two plan on the shaped passes: a pass of
+    2^a points runs a // 4 radix-16 stages (one fewer when a % 4 == 1)."""
+    import re
+
+    m = re.search(r"inner=(\d+) (four-step (\d+)x(\d+)|one-pass)", describe)
+    if not m:
+        return 0
+    lens = [int(m.group(3)), int(m.group(4))] if m.group(3) else [int(m.group(1))]
+    total = 0
+    for L in lens:
+        if L < 16 or L > 4096 or L & (L - 1):
+            return 0
+        a = L.bit_length() - 1
+        q, rr = divmod(a, 4)
+        total += q - 1 if (rr == 1 and q > 0) else q
+    return total
+
+
+def _gpu_form(form: str, n: int, batch: int, steps: int, warmup: int, x64, x32, flags: int = 0):
     import torch
 
     import kfft_amd as K
 
-    half = form == "fp32"
-    plan = K.CpxPlan(n, 0, half=half)
-    x = x64.to(torch.complex64) if half else x64
+    half = form != "fp64"
+    plan = K.CpxPlan(n, flags | _form_flags(form), half=half)
+    x = x32 if half else x64
     y = torch.empty_like(x)
     stream = torch.cuda.ExternalStream(K.load(half).kfft_gpu_stream())
     for _ in range(warmup):
@@ -103,9 +146,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1 << 16)
     ap.add_argument("--batch", type=int, default=4096)
-    ap.add_argument("--form", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--form", choices=list(FORMS), default="f16")
+    ap.add_argument("--forms", default=",".join(FORMS),
+                    help="comma-separated forms to run (the reported one is always included)")
     ap.add_argument("--cpu-sample", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--onchip", action="store_true",
+                    help="plan with KFFT_FLAG_GPU_ONCHIP (fused persistent two-pass kernel)")
     ap.add_argument("--copy-ref", action="store_true",
                     help="after timing, run one plain device copy of the FP64 input (4 GiB at the "
                          "default size) so that profiler byte counters can be calibrated")
@@ -120,28 +167,83 @@ def main():
     import torch
 
     n, batch = args.n, args.batch
+    forms = [f for f in FORMS if f in args.forms.split(",") or f in (args.form, "fp64")]
     g = torch.Generator(device="cuda").manual_seed(20260103)
-    x64 = torch.view_as_complex(
-        (torch.rand(batch, n, 2, generator=g, device="cuda", dtype=torch.float64) * 2 - 1)
+    # set A: uniform[-1,1) re/im, rounded to FP32 once so that every form sees the same values
+    x32 = torch.view_as_complex(
+        torch.rand(batch, n, 2, generator=g, device="cuda", dtype=torch.float32) * 2 - 1
     ).contiguous()
+    x64 = x32.to(torch.complex128)
 
+    import kfft_amd as K
+
+    flags = K.FLAG_GPU_ONCHIP if args.onchip else 0
     res = {}
-    for form in ("fp64", "fp32"):
-        plan, y, ms, wall = _gpu_form(form, n, batch, args.steps, args.warmup, x64)
-        res[form] = (plan, y, ms, wall)
-    # accuracy of the fp32 path against the fp64 path (all instances)
-    y64, y32 = res["fp64"][1], res["fp32"][1]
-    err32 = (torch.linalg.vector_norm(y32.to(torch.complex128) - y64) /
-             torch.linalg.vector_norm(y64)).item()
-    del y32
+    for form in forms:
+        plan, y, ms, wall = _gpu_form(form, n, batch, args.steps, args.warmup, x64, x32, flags)
+        if plan.error():
+            raise RuntimeError(f"{form}: the fused transform reported a failed dependency wait")
+        res[form] = {"plan": plan, "ms": ms, "y": y}
+        print(f"# {form}: {ms:.4f} ms/step ({plan.describe()})", file=sys.stderr, flush=True)
+
+    def rel_err(y, ref):
+        return (torch.linalg.vector_norm(y.to(torch.complex128) - ref) /
+                torch.linalg.vector_norm(ref)).item()
+
+    # accuracy on set A against the FP64 path
+    y64 = res["fp64"]["y"]
+    for form in forms:
+        if form != "fp64":
+            res[form]["err_uniform"] = rel_err(res[form]["y"], y64)
+        del res[form]["y"]
+    del y64
+    # set B: unit noise plus five 1e-6-amplitude tones per transform
+    t = torch.arange(n, device="cuda", dtype=torch.float64)
+    freqs = torch.randint(0, n, (batch, 5), generator=g, device="cuda")
+    xb = x64.clone()
+    for j in range(5):
+        ph = 2 * math.pi * (freqs[:, j:j + 1].to(torch.float64) * t[None, :] % n) / n
+        xb += 1e-6 * torch.polar(torch.ones_like(ph), ph)
+    xb32 = xb.to(torch.complex64)
+    xb = xb32.to(torch.complex128)
+    del x64, t, freqs
+    torch.cuda.synchronize()
+    yb64 = torch.empty_like(xb)
+    K.CpxPlan(n, 0)(xb, out=yb64)
+    yb = torch.empty_like(xb32)
+    for form in forms:
+        if form != "fp64":
+            K.CpxPlan(n, _form_flags(form), half=True)(xb32, out=yb)
+            res[form]["err_tones"] = rel_err(yb, yb64)
+    del yb, yb64, xb, xb32
 
     form = args.form
-    plan, _, ms, wall = res[form]
-    esz = 8 if form == "fp32" else 16
+    plan, ms = res[form]["plan"], res[form]["ms"]
+    esz = 8 if form != "fp64" else 16
     flops = 5.0 * n * math.log2(n) * batch
     value = flops / (ms * 1e-3) / 1e9
     alg_bytes = 2.0 * n * esz * batch
     achieved = alg_bytes / (ms * 1e-3) / 1e9
+    r16 = _radix16_stages(res["fp32"]["plan"].describe()) if "fp32" in res else 0
+
+    def form_entry(f, r):
+        e = {"ms_per_step": round(r["ms"], 4),
+             "gflops": round(flops / (r["ms"] * 1e-3) / 1e9, 2),
+             "hbm_frac": round(2.0 * n * (8 if f != "fp64" else 16) * batch /
+                               (r["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+             "speedup_vs_fp64": round(res["fp64"]["ms"] / r["ms"], 3)}
+        if "err_uniform" in r:
+            e["rel_l2_vs_fp64"] = {"uniform": r["err_uniform"], "tones_1e-6": r["err_tones"]}
+        if f[:3] in ("f16", "bf1"):
+            # per 64 butterflies: 2 groups x (3 products (F16) or 6 (BF16)) x 2 halves (Re, Im)
+            per_wave = 12 if f.startswith("f16") else 24
+            mf = r16 * (n // 16 // 64) * per_wave * (32 * 32 * 16 * 2) * batch
+            passes = 2 if f.endswith("r") else 1
+            e["mfma"] = {"flop_per_step": mf * passes,
+                         "util": round(mf * passes / (r["ms"] * 1e-3) / MFMA_PEAK_FLOPS, 4),
+                         "radix16_stages": r16}
+        return e
+
     out = {
         "metric": "batched C2C GFLOP/s (5N*log2N/t) & %HBM peak, FP32-split vs FP64 at <=1e-6",
         "value": round(value, 2),
@@ -153,21 +255,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if form == "fp32" else "f64",
+        "dtype": "f32" if form != "fp64" else "f64",
         "data": "synthetic uniform[-1,1) seeded, resident in HBM",
         "config": {"workload": f"batched 1D C2C forward N={n} batch={batch}", "n": n,
-                   "batch": batch, "form": form + "-plain", "plan": plan.describe()},
+                   "batch": batch, "form": form, "plan": plan.describe(),
+                   "onchip": bool(args.onchip)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "whole transform (all passes of one step)",
                      "alg_bytes_per_step": alg_bytes},
-        "forms": {f: {"ms_per_step": round(r[2], 4),
-                      "gflops": round(flops / (r[2] * 1e-3) / 1e9, 2),
-                      "hbm_frac": round(2.0 * n * (8 if f == "fp32" else 16) * batch /
-                                        (r[2] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-                  for f, r in res.items()},
-        "fp32_rel_l2_vs_fp64": err32,
-        "fp32_speedup_vs_fp64": round(res["fp64"][2] / res["fp32"][2], 3),
+        "forms": {f: form_entry(f, r) for f, r in res.items()},
     }
     if args.copy_ref:
         ref = torch.empty_like(x64)
@@ -175,7 +272,7 @@ def main():
         torch.cuda.synchronize()
         out["copy_ref_bytes"] = 2 * x64.numel() * x64.element_size()
         del ref
-    traffic = _measured_traffic(form, n, batch)
+    traffic = None if args.onchip else _measured_traffic(form, n, batch)
     if traffic is not None:
         out["roofline"]["traffic"] = traffic
     if not args.no_cpu:

@@ -37,6 +37,8 @@ struct GpuObj {
 
 struct CpxObj : GpuObj {
     std::shared_ptr<Plan1D> p;
+    std::shared_ptr<Plan1D> p64; // refinement (float build, KFFT_FLAG_GPU_REFINE): FP64 plan
+    DevBuf ref_z, ref_r, ref_x, ref_scr;
 };
 
 struct SclrObj : GpuObj {
@@ -177,6 +179,22 @@ static bool is_device_ptr(const void* p) {
     return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
 }
 
+// Stream of a synchronous entry point: the library stream, ordered after all
+// work queued so far on the null stream, so that device buffers a caller (or a
+// framework on the null stream) has just produced are complete when read.
+static hipStream_t entry_stream() {
+    hipStream_t s = default_stream();
+    thread_local hipEvent_t ev = nullptr;
+    if (!ev && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        ev = nullptr;
+        (void)hipStreamSynchronize(nullptr);
+        return s;
+    }
+    if (hipEventRecord(ev, nullptr) != hipSuccess || hipStreamWaitEvent(s, ev, 0) != hipSuccess)
+        (void)hipStreamSynchronize(nullptr);
+    return s;
+}
+
 static kfft_return_t gpu_fail(const char* what) {
     if (what)
         set_error(std::string(what) + ": " + last_error());
@@ -271,6 +289,18 @@ static void host_twiddles(kfft_cpx* tw, uint32_t n, bool inverse) {
     }
 }
 
+// radix-16 arithmetic of the precision form (float build only; the double
+// build's data is FP64 and always runs the FP64 passes)
+static uint32_t split_dft(uint32_t flags) {
+    if (PREC != 4)
+        return 0;
+    if (flags & KFFT_FLAG_GPU_SPLIT_F16)
+        return KFA_FAST_MFMA_F16;
+    if (flags & KFFT_FLAG_GPU_SPLIT_BF16)
+        return KFA_FAST_MFMA_BF16;
+    return 0;
+}
+
 static double cpx_scale(const kfft_plan_cpx* P) {
     if ((P->flags & KFFT_FLAG_INVERSE) && !(P->flags & KFFT_FLAG_DISABLE_NORM))
         return 1.0 / (double)P->nfft;
@@ -288,6 +318,49 @@ static Layout contiguous(const void* p, int64_t dist, uint32_t flags = 0) {
     return l;
 }
 
+// The precision form with one refinement step (float build): y0 = T(x) in
+// FP32, the residual r = x - T^-1(y0) with T^-1 evaluated in FP64, then
+// y = y0 + T(r).  T^-1 of a forward plan is the inverse scaled by 1/N; of a
+// normalised inverse, the forward; of an unnormalised inverse, the forward
+// scaled by 1/N.  Runs in chunks of about 256 MiB of FP32 data.
+static kfft_return_t cpx_refine(kfft_plan_cpx* P, CpxObj* g, const void* din, void* dout,
+                                uint32_t batch, hipStream_t s, const Fuse& fz) {
+    const uint32_t n = P->nfft;
+    const bool inverse = (P->flags & KFFT_FLAG_INVERSE) != 0;
+    const double inv_scale =
+        (!inverse || (P->flags & KFFT_FLAG_DISABLE_NORM)) ? 1.0 / (double)n : 1.0;
+    const size_t inst32 = (size_t)n * 8, inst64 = (size_t)n * 16;
+    const uint32_t chunk =
+        (uint32_t)std::max<size_t>(1, std::min<size_t>(batch, ((size_t)256 << 20) / inst32));
+    const bool alias = din == dout;
+    if (!g->scratch.ensure(std::max(g->p->scratch_bytes(chunk), g->p64->scratch_bytes(chunk))) ||
+        !g->ref_z.ensure(inst64 * chunk) || !g->ref_r.ensure(inst32 * chunk) ||
+        (alias && !g->ref_x.ensure(inst32 * chunk)))
+        return gpu_fail("refinement buffers");
+    for (uint32_t c0 = 0; c0 < batch; c0 += chunk) {
+        const uint32_t nb = std::min(chunk, batch - c0);
+        const uint64_t cnt = (uint64_t)n * nb;
+        const char* x = (const char*)din + (size_t)c0 * inst32;
+        char* y = (char*)dout + (size_t)c0 * inst32;
+        if (alias) {
+            if (!hip_ok(launch_copy(4, x, g->ref_x.p, cnt, 0, 1.0, s), "refine save"))
+                return KFFT_RET_GPU_FAIL;
+            x = (const char*)g->ref_x.p;
+        }
+        if (!hip_ok(g->p->execute(contiguous(x, n), contiguous(y, n), nb, inverse, cpx_scale(P),
+                                  g->scratch.p, s, &fz), "refine y0") ||
+            !hip_ok(launch_widen(y, g->ref_z.p, cnt, s), "refine widen") ||
+            !hip_ok(g->p64->execute(contiguous(g->ref_z.p, n), contiguous(g->ref_z.p, n), nb,
+                                    !inverse, inv_scale, g->scratch.p, s), "refine residual fft") ||
+            !hip_ok(launch_residual(x, g->ref_z.p, g->ref_r.p, cnt, s), "refine residual") ||
+            !hip_ok(g->p->execute(contiguous(g->ref_r.p, n), contiguous(g->ref_r.p, n), nb, inverse,
+                                  cpx_scale(P), g->scratch.p, s, &fz), "refine correction") ||
+            !hip_ok(launch_accumulate(y, g->ref_r.p, cnt, s), "refine add"))
+            return KFFT_RET_GPU_FAIL;
+    }
+    return KFFT_RET_SUCCESS;
+}
+
 // Batched complex transform on device pointers, on stream s, no sync.
 static kfft_return_t cpx_run_dev(kfft_plan_cpx* P, const void* din, void* dout, uint32_t batch,
                                  hipStream_t s) {
@@ -295,13 +368,16 @@ static kfft_return_t cpx_run_dev(kfft_plan_cpx* P, const void* din, void* dout, 
     if (!g || !g->p)
         return KFFT_RET_BADARGUMENTS;
     const Plan1D& pl = *g->p;
-    if (!g->scratch.ensure(pl.scratch_bytes(batch)))
-        return gpu_fail("scratch");
     Fuse fz;
     fz.onchip = (P->flags & KFFT_FLAG_GPU_ONCHIP) != 0;
-    hipError_t e =
-        pl.execute(contiguous(din, P->nfft), contiguous(dout, P->nfft), batch,
-                   (P->flags & KFFT_FLAG_INVERSE) != 0, cpx_scale(P), g->scratch.p, s, &fz);
+    fz.dft = split_dft(P->flags);
+    const bool inverse = (P->flags & KFFT_FLAG_INVERSE) != 0;
+    if (g->p64)
+        return cpx_refine(P, g, din, dout, batch, s, fz);
+    if (!g->scratch.ensure(pl.scratch_bytes(batch)))
+        return gpu_fail("scratch");
+    hipError_t e = pl.execute(contiguous(din, P->nfft), contiguous(dout, P->nfft), batch, inverse,
+                              cpx_scale(P), g->scratch.p, s, &fz);
     return hip_ok(e, "cpx execute") ? KFFT_RET_SUCCESS : KFFT_RET_GPU_FAIL;
 }
 
@@ -313,7 +389,7 @@ static kfft_return_t cpx_run(kfft_plan_cpx* P, const kfft_cpx* fin, kfft_cpx* fo
     if (!g)
         return KFFT_RET_BADARGUMENTS;
     std::lock_guard<std::mutex> lk(g->mu);
-    hipStream_t s = default_stream();
+    hipStream_t s = entry_stream();
     const size_t bytes = (size_t)P->nfft * CPXB * batch;
     Staged si, so;
     if (!stage_in(fin, bytes, g->stage_in, si, s) || !stage_out(fout, bytes, g->stage_out, so))
@@ -352,7 +428,9 @@ static kfft_plan_cpx* config_cpx(uint32_t nfft, uint32_t flags, kfft_pool_t* A, 
         return nullptr;
     }
     g->p = get_plan(nfft, PREC);
-    if (!g->p) {
+    if (PREC == 4 && (flags & KFFT_FLAG_GPU_REFINE))
+        g->p64 = get_plan(nfft, 8);
+    if (!g->p || (PREC == 4 && (flags & KFFT_FLAG_GPU_REFINE) && !g->p64)) {
         delete g;
         plan_terminate(P, A);
         return nullptr;
@@ -485,7 +563,7 @@ static kfft_return_t sclr_fwd(kfft_plan_sclr* P, const kfft_scalar* fin, kfft_cp
         return KFFT_RET_IMPROPER_PLAN;
     SclrObj* g = (SclrObj*)P->gpu;
     std::lock_guard<std::mutex> lk(g->mu);
-    hipStream_t s = default_stream();
+    hipStream_t s = entry_stream();
     const uint32_t ol = sclr_out_len(P);
     Staged si, so;
     if (!stage_in(fin, (size_t)P->nfft * sizeof(kfft_scalar) * batch, g->stage_in, si, s) ||
@@ -508,7 +586,7 @@ static kfft_return_t sclr_inv(kfft_plan_sclr* P, const kfft_cpx* fin, kfft_scala
         return KFFT_RET_IMPROPER_PLAN;
     SclrObj* g = (SclrObj*)P->gpu;
     std::lock_guard<std::mutex> lk(g->mu);
-    hipStream_t s = default_stream();
+    hipStream_t s = entry_stream();
     const uint32_t il = sclr_out_len(P);
     Staged si, so;
     if (!stage_in(fin, (size_t)il * CPXB * batch, g->stage_in, si, s) ||
@@ -607,7 +685,7 @@ static kfft_return_t c2d_run(kfft_plan_c2d* P, const kfft_cpx* fin, kfft_cpx* fo
         return KFFT_RET_BADARGUMENTS;
     C2dObj* g = (C2dObj*)P->gpu;
     std::lock_guard<std::mutex> lk(g->mu);
-    hipStream_t s = default_stream();
+    hipStream_t s = entry_stream();
     const size_t bytes = (size_t)P->nfft * CPXB;
     Staged si, so;
     if (!stage_in(fin, bytes, g->stage_in, si, s) || !stage_out(fout, bytes, g->stage_out, so))
@@ -924,7 +1002,7 @@ KFFT_API kfft_return_t kfft_eval2_scalar(kfft_plan_s2d* plan, const kfft_scalar*
         return KFFT_RET_IMPROPER_PLAN;
     S2dObj* g = (S2dObj*)plan->gpu;
     std::lock_guard<std::mutex> lk(g->mu);
-    hipStream_t s = default_stream();
+    hipStream_t s = entry_stream();
     Staged si, so;
     if (!stage_in(fin, (size_t)plan->nfft * sizeof(kfft_scalar), g->stage_in, si, s) ||
         !stage_out(fout, (size_t)plan->nfft * CPXB, g->stage_out, so))
@@ -944,7 +1022,7 @@ KFFT_API kfft_return_t kfft_evali2_scalar(kfft_plan_s2d* plan, const kfft_cpx* f
         return KFFT_RET_IMPROPER_PLAN;
     S2dObj* g = (S2dObj*)plan->gpu;
     std::lock_guard<std::mutex> lk(g->mu);
-    hipStream_t s = default_stream();
+    hipStream_t s = entry_stream();
     Staged si, so;
     if (!stage_in(fin, (size_t)plan->nfft * CPXB, g->stage_in, si, s) ||
         !stage_out(fout, (size_t)plan->nfft * sizeof(kfft_scalar), g->stage_out, so))
@@ -988,7 +1066,7 @@ KFFT_API kfft_return_t kfft_eval_conv_cpx(kfft_plan_ccnv* plan, const kfft_cpx* 
         return KFFT_RET_BADARGUMENTS;
     ConvObj* g = (ConvObj*)plan->gpu;
     std::lock_guard<std::mutex> lk(g->mu);
-    hipStream_t s = default_stream();
+    hipStream_t s = entry_stream();
     const uint32_t n = plan->nfft;
     const size_t bytes = (size_t)n * CPXB;
     Staged sa, sb, so;
@@ -1053,7 +1131,7 @@ KFFT_API kfft_return_t kfft_eval_conv_scalar(kfft_plan_scnv* plan, const kfft_sc
         return KFFT_RET_BADARGUMENTS;
     ConvObj* g = (ConvObj*)plan->gpu;
     std::lock_guard<std::mutex> lk(g->mu);
-    hipStream_t s = default_stream();
+    hipStream_t s = entry_stream();
     const uint32_t n = plan->nfft, ol = sclr_out_len(plan->plan_fwd);
     const size_t rb = (size_t)n * sizeof(kfft_scalar), cb = (size_t)ol * CPXB;
     Staged sa, sb, so;
@@ -1109,7 +1187,7 @@ KFFT_API kfft_return_t kfft_eval2_conv_cpx(kfft_plan_c2cnv* plan, const kfft_cpx
         return KFFT_RET_BADARGUMENTS;
     ConvObj* g = (ConvObj*)plan->gpu;
     std::lock_guard<std::mutex> lk(g->mu);
-    hipStream_t s = default_stream();
+    hipStream_t s = entry_stream();
     const size_t bytes = (size_t)plan->nfft * CPXB;
     Staged sa, sb, so;
     if (!stage_in(fin_A, bytes, g->stage_in, sa, s) || !stage_in(fin_B, bytes, g->stage_in2, sb, s) ||
@@ -1161,7 +1239,7 @@ KFFT_API kfft_return_t kfft_eval2_conv_scalar(kfft_plan_s2cnv* plan, const kfft_
         return KFFT_RET_BADARGUMENTS;
     ConvObj* g = (ConvObj*)plan->gpu;
     std::lock_guard<std::mutex> lk(g->mu);
-    hipStream_t s = default_stream();
+    hipStream_t s = entry_stream();
     const size_t rb = (size_t)plan->nfft * sizeof(kfft_scalar), cb = (size_t)plan->nfft * CPXB;
     Staged sa, sb, so;
     if (!stage_in(fin_A, rb, g->stage_in, sa, s) || !stage_in(fin_B, rb, g->stage_in2, sb, s) ||
@@ -1183,7 +1261,7 @@ KFFT_API kfft_return_t kfft_eval2_conv_scalar(kfft_plan_s2cnv* plan, const kfft_
 // ------------------------------------------------------------------- math
 KFFT_API void kfft_math_hadamard_cpx(kfft_cpx* Fout, const kfft_cpx* Fin, uint32_t size) {
     if (is_device_ptr(Fout) && is_device_ptr(Fin)) {
-        hipStream_t s = default_stream();
+        hipStream_t s = entry_stream();
         hadamard_dev(Fout, Fin, size, s);
         (void)hipStreamSynchronize(s);
         return;
@@ -1197,7 +1275,7 @@ KFFT_API void kfft_math_hadamard_cpx(kfft_cpx* Fout, const kfft_cpx* Fin, uint32
 
 KFFT_API void kfft_math_hadamard_scalar(kfft_scalar* Fout, const kfft_scalar* Fin, uint32_t size) {
     if (is_device_ptr(Fout) && is_device_ptr(Fin)) {
-        hipStream_t s = default_stream();
+        hipStream_t s = entry_stream();
         (void)hip_ok(launch_hadamard_real(PREC, Fout, Fin, size, s), "hadamard");
         (void)hipStreamSynchronize(s);
         return;
@@ -1210,7 +1288,7 @@ KFFT_API void kfft_math_hadamard_scalar(kfft_scalar* Fout, const kfft_scalar* Fi
 KFFT_API void kfft_math_transpose_cpx(const kfft_cpx* Fin, kfft_cpx* Fout, const uint32_t x,
                                       const uint32_t y) {
     if (is_device_ptr(Fin) && is_device_ptr(Fout)) {
-        hipStream_t s = default_stream();
+        hipStream_t s = entry_stream();
         (void)hip_ok(launch_transpose((int)CPXB, Fin, Fout, x, y, 1, s), "transpose");
         (void)hipStreamSynchronize(s);
         return;
@@ -1222,7 +1300,7 @@ KFFT_API void kfft_math_transpose_cpx(const kfft_cpx* Fin, kfft_cpx* Fout, const
 KFFT_API void kfft_math_transpose_scalar(const kfft_scalar* Fin, kfft_scalar* Fout,
                                          const uint32_t x, const uint32_t y) {
     if (is_device_ptr(Fin) && is_device_ptr(Fout)) {
-        hipStream_t s = default_stream();
+        hipStream_t s = entry_stream();
         (void)hip_ok(launch_transpose((int)sizeof(kfft_scalar), Fin, Fout, x, y, 1, s), "transpose");
         (void)hipStreamSynchronize(s);
         return;

@@ -226,6 +226,28 @@ __global__ void k_copy(const cx<T>* in, cx<T>* out, uint64_t n, int conj_, doubl
     }
 }
 
+// Refinement of the precision form (FP32 data, FP64 residual):
+//   z = widen(y0);  z = T^-1(z) in FP64;  r = x - z (rounded once);  y = y0 + T(r)
+__global__ void k_widen(const cx<float>* in, cx<double>* out, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = {(double)in[i].re, (double)in[i].im};
+}
+
+__global__ void k_residual(const cx<float>* x, const cx<double>* z, cx<float>* r, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        r[i] = {(float)((double)x[i].re - z[i].re), (float)((double)x[i].im - z[i].im)};
+}
+
+__global__ void k_accumulate(cx<float>* y, const cx<float>* d, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const cx<float> a = y[i], b = d[i];
+        y[i] = {a.re + b.re, a.im + b.im};
+    }
+}
+
 // out[b] = x_b[0] for every instance b (the Rader x[0] values), so that passes
 // writing the output in place never race with readers of x[0].
 template <typename T>
@@ -432,6 +454,30 @@ hipError_t launch_copy(int prec, const void* in, void* out, uint64_t n, int conj
     else
         hipLaunchKernelGGL(k_copy<float>, dim3(blocks_for(n, 256)), dim3(256), 0, s,
                            (const cx<float>*)in, (cx<float>*)out, n, conj_, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_widen(const void* in, void* out, uint64_t n, hipStream_t s) {
+    if (!n)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_widen, dim3(blocks_for(n, 256)), dim3(256), 0, s, (const cx<float>*)in,
+                       (cx<double>*)out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual(const void* x, const void* z, void* r, uint64_t n, hipStream_t s) {
+    if (!n)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_residual, dim3(blocks_for(n, 256)), dim3(256), 0, s, (const cx<float>*)x,
+                       (const cx<double>*)z, (cx<float>*)r, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_accumulate(void* y, const void* d, uint64_t n, hipStream_t s) {
+    if (!n)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_accumulate, dim3(blocks_for(n, 256)), dim3(256), 0, s, (cx<float>*)y,
+                       (const cx<float>*)d, n);
     return hipGetLastError();
 }
 

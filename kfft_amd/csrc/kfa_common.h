@@ -77,6 +77,8 @@ struct KfaFftArgs {
 #define KFA_FAST_TW4 8u
 #define KFA_FAST_NT_IN 16u  // non-temporal (streaming) loads of the input
 #define KFA_FAST_NT_OUT 32u // non-temporal (streaming) stores of the output
+#define KFA_FAST_MFMA_F16 64u   // radix-16 stages as split FP16 matrix products (FP32 data)
+#define KFA_FAST_MFMA_BF16 128u // radix-16 stages as split BF16 matrix products (FP32 data)
 
 struct FastArgs {
     const void* in;

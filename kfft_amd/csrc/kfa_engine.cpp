@@ -367,7 +367,7 @@ static hipError_t run_pass(const LdsPass& P, const KfaIo& in, const KfaIo& out, 
 // Both passes of the fast two-pass transform for `batch` contiguous instances.
 hipError_t SubFft::run_four_fast(const KfaIo& load, const KfaIo& store, double scale,
                                  uint64_t batch, void* scratch, uint32_t nt,
-                                 hipStream_t s) const {
+                                 hipStream_t s, uint32_t dft) const {
     // pass A: columns n2 of length N1 (stride N2), W_L^(n2 k1) on store
     FastArgs a = fast_args(col.tw.p, batch * N2, 1.0);
     a.in = (const void*)load.ptr;
@@ -382,7 +382,7 @@ hipError_t SubFft::run_four_fast(const KfaIo& load, const KfaIo& store, double s
     a.out_stride = N2;
     a.tw4 = tw4_full.p;
     a.flags = KFA_FAST_TW4 | ((load.flags & KFA_IO_CONJ1) ? KFA_FAST_CONJ_IN : 0) |
-              ((nt & 1) ? KFA_FAST_NT_IN : 0) | ((nt & 4) ? KFA_FAST_NT_OUT : 0);
+              ((nt & 1) ? KFA_FAST_NT_IN : 0) | ((nt & 4) ? KFA_FAST_NT_OUT : 0) | dft;
     hipError_t e = launch_fast(prec, N1, 1, a, s);
     if (e != hipSuccess)
         return e;
@@ -400,7 +400,7 @@ hipError_t SubFft::run_four_fast(const KfaIo& load, const KfaIo& store, double s
     b.out_stride = (int64_t)N1 * store.stride;
     b.flags = ((store.flags & KFA_IO_CONJ2) ? KFA_FAST_CONJ_OUT : 0) |
               ((store.flags & KFA_IO_SCALE) ? KFA_FAST_SCALE : 0) |
-              ((nt & 2) ? KFA_FAST_NT_OUT : 0) | ((nt & 8) ? KFA_FAST_NT_IN : 0);
+              ((nt & 2) ? KFA_FAST_NT_OUT : 0) | ((nt & 8) ? KFA_FAST_NT_IN : 0) | dft;
     return launch_fast(prec, N2, 2, b, s);
 }
 
@@ -501,7 +501,8 @@ int SubFft::fused_error() const {
 }
 
 hipError_t SubFft::run(const KfaIo& load, const KfaIo& store, double scale, double scale_post,
-                       uint64_t batch, void* scratch, hipStream_t s, bool onchip) const {
+                       uint64_t batch, void* scratch, hipStream_t s, bool onchip,
+                       uint32_t dft) const {
     const bool simple = fast_io_ok(load, KFA_IO_CONJ1) &&
                         fast_io_ok(store, KFA_IO_CONJ2 | KFA_IO_SCALE) && scale_post == 1.0;
     if (!four) {
@@ -523,7 +524,7 @@ hipError_t SubFft::run(const KfaIo& load, const KfaIo& store, double scale, doub
                 f.out_stride = store.stride;
                 f.flags = ((load.flags & KFA_IO_CONJ1) ? KFA_FAST_CONJ_IN : 0) |
                           ((store.flags & KFA_IO_CONJ2) ? KFA_FAST_CONJ_OUT : 0) |
-                          ((store.flags & KFA_IO_SCALE) ? KFA_FAST_SCALE : 0);
+                          ((store.flags & KFA_IO_SCALE) ? KFA_FAST_SCALE : 0) | dft;
                 return launch_fast(prec, L, rows ? 0 : 1, f, s);
             }
         }
@@ -531,7 +532,7 @@ hipError_t SubFft::run(const KfaIo& load, const KfaIo& store, double scale, doub
     }
     if (load.C != 1 || store.C != 1)
         return hipErrorInvalidValue;
-    if (onchip && simple && tw4_full.p && N1 == 256 && N2 == 256 && s == default_stream() &&
+    if (onchip && !dft && simple && tw4_full.p && N1 == 256 && N2 == 256 && s == default_stream() &&
         load.stride == 1 && store.stride == 1 && fused_queues() > 0)
         return run_fused(load, store, scale, batch, s);
     if (simple && tw4_full.p) {
@@ -547,7 +548,7 @@ hipError_t SubFft::run(const KfaIo& load, const KfaIo& store, double scale, doub
             KfaIo ld = load, st = store;
             ld.ptr += (uint64_t)((int64_t)c0 * load.dist_hi * esz);
             st.ptr += (uint64_t)((int64_t)c0 * store.dist_hi * esz);
-            hipError_t e = run_four_fast(ld, st, scale, nb, scratch, tn.nt, s);
+            hipError_t e = run_four_fast(ld, st, scale, nb, scratch, tn.nt, s, dft);
             if (e != hipSuccess)
                 return e;
         }
@@ -812,11 +813,13 @@ hipError_t Plan1D::execute(const Layout& in, const Layout& out, uint64_t batch, 
             store.flags |= KFA_IO_CONJ2;
         if (scaled)
             store.flags |= KFA_IO_SCALE;
-        return sub_.run(load, store, scale, 1.0, batch, scratch, s, fuse && fuse->onchip);
+        return sub_.run(load, store, scale, 1.0, batch, scratch, s, fuse && fuse->onchip,
+                        fuse ? fuse->dft : 0u);
     }
     if (fuse && fuse->pw_in)
         return hipErrorInvalidValue; // callers apply the product separately for these kinds
     const uint32_t L = sub_.L;
+    const uint32_t dft = fuse ? fuse->dft : 0u;
     char* sp = (char*)scratch;
     if (kind_ == Kind::Rader) {
         const bool real_in = (in.flags & KFA_IO_REAL) != 0;
@@ -835,7 +838,7 @@ hipError_t Plan1D::execute(const Layout& in, const Layout& out, uint64_t batch, 
         l1.flags |= KFA_IO_MAP | (inverse ? KFA_IO_CONJ1 : 0);
         l1.map = (uint64_t)qidx_.p;
         KfaIo s1 = io_plain(tmpA, L);
-        e = sub_.run(l1, s1, 1.0, 1.0, batch, sub_scr, s);
+        e = sub_.run(l1, s1, 1.0, 1.0, batch, sub_scr, s, false, dft);
         if (e != hipSuccess)
             return e;
         // pass 2: c = IFFT(A * FFT(b)); X[g^-m] = x0 + c[m]
@@ -850,7 +853,7 @@ hipError_t Plan1D::execute(const Layout& in, const Layout& out, uint64_t batch, 
         s2.aux = (uint64_t)x0;
         s2.aux_dist = (s2.C > 1) ? s2.C : 1; // x0 is indexed by instance = hi*C + lo
         s2.aux_dist_lo = (s2.C > 1) ? 1 : 0;
-        e = sub_.run(l2, s2, 1.0 / (double)L, scale, batch, sub_scr, s);
+        e = sub_.run(l2, s2, 1.0 / (double)L, scale, batch, sub_scr, s, false, dft);
         if (e != hipSuccess)
             return e;
         // X[0] = x0 + A[0]
@@ -866,7 +869,7 @@ hipError_t Plan1D::execute(const Layout& in, const Layout& out, uint64_t batch, 
     l1.pw = (uint64_t)chirp_.p;
     l1.valid = n_;
     KfaIo s1 = io_plain(tmp, L);
-    hipError_t e = sub_.run(l1, s1, 1.0, 1.0, batch, sub_scr, s);
+    hipError_t e = sub_.run(l1, s1, 1.0, 1.0, batch, sub_scr, s, false, dft);
     if (e != hipSuccess)
         return e;
     KfaIo l2 = io_plain(tmp, L);
@@ -876,7 +879,7 @@ hipError_t Plan1D::execute(const Layout& in, const Layout& out, uint64_t batch, 
     s2.flags |= KFA_IO_CONJ1 | KFA_IO_PW | KFA_IO_SCALE | (inverse ? KFA_IO_CONJ2 : 0);
     s2.pw = (uint64_t)chirp_.p;
     s2.valid = n_;
-    return sub_.run(l2, s2, scale / (double)L, 1.0, batch, sub_scr, s);
+    return sub_.run(l2, s2, scale / (double)L, 1.0, batch, sub_scr, s, false, dft);
 }
 
 // =============================================================== plan registry

@@ -63,6 +63,7 @@ struct Fuse {
     const void* pw_in = nullptr; // pointwise multiply on load, same layout as the input
     bool pw_in_f64 = false;
     bool onchip = false;         // allow the fused on-chip two-pass kernel
+    uint32_t dft = 0;            // KFA_FAST_MFMA_F16 / _BF16: split-operand radix-16 stages
 };
 
 // -------------------------------------------------------------- LDS pass plans
@@ -100,10 +101,12 @@ struct SubFft {
     size_t scratch_elems(uint64_t batch) const { return four ? (size_t)batch * L : 0; }
     // Run with fully described first-load and last-store descriptors.  For
     // four-step the load/store must have C == 1 (contiguous instances).
+    // `dft` (KFA_FAST_MFMA_* bits) selects the radix-16 arithmetic of the fast passes
     hipError_t run(const KfaIo& load, const KfaIo& store, double scale, double scale_post,
-                   uint64_t batch, void* scratch, hipStream_t s, bool onchip = false) const;
+                   uint64_t batch, void* scratch, hipStream_t s, bool onchip = false,
+                   uint32_t dft = 0) const;
     hipError_t run_four_fast(const KfaIo& load, const KfaIo& store, double scale, uint64_t batch,
-                             void* scratch, uint32_t nt, hipStream_t s) const;
+                             void* scratch, uint32_t nt, hipStream_t s, uint32_t dft) const;
     // fused on-chip variant of the two passes (N1 == N2 == 256, library stream only)
     hipError_t run_fused(const KfaIo& load, const KfaIo& store, double scale, uint64_t batch,
                          hipStream_t s) const;

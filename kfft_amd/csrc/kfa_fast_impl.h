@@ -17,7 +17,10 @@
 #pragma once
 #include <stdlib.h>
 
+#include <type_traits>
+
 #include "kfa_device.h"
+#include "kfa_mfma.h"
 
 namespace kfa {
 
@@ -85,12 +88,34 @@ __device__ __forceinline__ void store_nt(cx<T>* p, cx<T> x) {
     __builtin_nontemporal_store(v, reinterpret_cast<V*>(p));
 }
 
+// One radix-R butterfly; radix-16 goes to the matrix cores when DK selects a
+// split-operand form (FP32 data, whole waves only).
+template <typename T, int R, int DK>
+__device__ __forceinline__ void dft_any(cx<T>* v, const MfmaConst<DK>& K) {
+    if constexpr (R == 16 && DK != DFT_VALU && std::is_same<T, float>::value)
+        dft16_mfma<DK>(v, K);
+    else
+        dft<T, R>(v);
+}
+
+// DFT kind a launch may use: the split forms need FP32 data and workgroups of
+// whole waves (the matrix instruction spans all 64 lanes)
+template <typename T, int L, int IPW, int DK>
+constexpr int dft_kind() {
+    return (DK != DFT_VALU && std::is_same<T, float>::value && (IPW * P2<L>::TPI) % 64 == 0)
+               ? DK : DFT_VALU;
+}
+
 // One workgroup tile: instances [first, first + IPW) of the pass described by A.
 // The caller provides the LDS area (IPW * SP elements).
-template <typename T, int L, int MODE, int IPW>
+template <typename T, int L, int MODE, int IPW, int DK = DFT_VALU>
 __device__ __forceinline__ void fast_tile(const FastArgs& A, uint64_t first, cx<T>* lds) {
     using S = P2<L>;
     constexpr int E = S::E, TPI = S::TPI, NST = S::S;
+    constexpr int DKE = dft_kind<T, L, IPW, DK>();
+    MfmaConst<DKE> K;
+    if constexpr (DKE != DFT_VALU)
+        K = mfma_dft16_const<DKE>();
     constexpr int PSH = sizeof(T) == 8 ? KFA_PAD_SHIFT_F64 : KFA_PAD_SHIFT_F32;
     constexpr int SP = (S::PAD_L(PSH)) | 1; // odd LDS stride per instance
     const uint32_t tid = threadIdx.x;
@@ -129,7 +154,7 @@ __device__ __forceinline__ void fast_tile(const FastArgs& A, uint64_t first, cx<
             }
 #pragma unroll
         for (int i = 0; i < NB; ++i)
-            dft<T, R>(v + i * R);
+            dft_any<T, R, DKE>(v + i * R, K);
         if constexpr (NST > 1) {
             cx<T>* my = lds + (size_t)li * SP;
 #pragma unroll
@@ -168,7 +193,7 @@ __device__ __forceinline__ void fast_tile(const FastArgs& A, uint64_t first, cx<
         if (R == 16) {
 #pragma unroll
             for (int i = 0; i < E / 16; ++i)
-                dft<T, 16>(v + i * 16);
+                dft_any<T, 16, DKE>(v + i * 16, K);
         } else if (R == 8) {
 #pragma unroll
             for (int i = 0; i < E / 8; ++i)
@@ -229,10 +254,11 @@ __device__ __forceinline__ void fast_tile(const FastArgs& A, uint64_t first, cx<
     }
 }
 
-template <typename T, int L, int MODE, int IPW>
+template <typename T, int L, int MODE, int IPW, int DK>
 __global__ void __launch_bounds__(IPW* P2<L>::TPI) k_fast(const FastArgs A) {
     extern __shared__ __align__(16) unsigned char kfa_smem[];
-    fast_tile<T, L, MODE, IPW>(A, (uint64_t)blockIdx.x * IPW, reinterpret_cast<cx<T>*>(kfa_smem));
+    fast_tile<T, L, MODE, IPW, DK>(A, (uint64_t)blockIdx.x * IPW,
+                                   reinterpret_cast<cx<T>*>(kfa_smem));
 }
 
 template <int L, int MODE, typename T, int MUL>
@@ -255,9 +281,10 @@ constexpr int fast_ipw() {
     return w < thr ? w : thr;
 }
 
-template <typename T, int L, int MODE, int MUL>
+template <typename T, int L, int MODE, int MUL, int DK>
 static hipError_t launch_fast_t(const FastArgs& a, hipStream_t s) {
     constexpr int IPW = fast_ipw<L, MODE, T, MUL>();
+    constexpr int DKE = dft_kind<T, L, IPW, DK>();
     constexpr int PSH = sizeof(T) == 8 ? KFA_PAD_SHIFT_F64 : KFA_PAD_SHIFT_F32;
     constexpr int SP = (P2<L>::PAD_L(PSH)) | 1;
     const uint32_t lds = P2<L>::S > 1 ? (uint32_t)(IPW * SP * sizeof(cx<T>)) : 0u;
@@ -265,7 +292,7 @@ static hipError_t launch_fast_t(const FastArgs& a, hipStream_t s) {
     static_assert(IPW * P2<L>::TPI <= 1024, "workgroup size");
     static bool attr = false;
     if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_fast<T, L, MODE, IPW>,
+        hipError_t e = hipFuncSetAttribute((const void*)k_fast<T, L, MODE, IPW, DKE>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         if (e != hipSuccess) {
             (void)hipGetLastError(); // do not leave the error for the next launch
@@ -276,7 +303,7 @@ static hipError_t launch_fast_t(const FastArgs& a, hipStream_t s) {
     const uint64_t groups = (a.batch + IPW - 1) / IPW;
     if (!groups)
         return hipSuccess;
-    hipLaunchKernelGGL((k_fast<T, L, MODE, IPW>), dim3((unsigned)groups),
+    hipLaunchKernelGGL((k_fast<T, L, MODE, IPW, DKE>), dim3((unsigned)groups),
                        dim3(IPW * P2<L>::TPI), lds, s, a);
     return hipGetLastError();
 }
@@ -292,41 +319,54 @@ static inline int fast_cols_mul() {
     return m;
 }
 
-template <typename T, int L, int MODE>
+template <typename T, int L, int MODE, int DK>
 static hipError_t launch_fast_sel(const FastArgs& a, hipStream_t s) {
     if constexpr (MODE != FAST_ROW && L >= 128 && L <= 1024) {
         const int m = fast_cols_mul();
         if (m == 2)
-            return launch_fast_t<T, L, MODE, 2>(a, s);
+            return launch_fast_t<T, L, MODE, 2, DK>(a, s);
         if (m == 4)
-            return launch_fast_t<T, L, MODE, 4>(a, s);
+            return launch_fast_t<T, L, MODE, 4, DK>(a, s);
     }
-    return launch_fast_t<T, L, MODE, 1>(a, s);
+    return launch_fast_t<T, L, MODE, 1, DK>(a, s);
 }
 
-template <typename T, int MODE>
+template <typename T, int MODE, int DK>
 static hipError_t launch_fast_mode(uint32_t L, const FastArgs& a, hipStream_t s) {
     switch (L) {
-    case 16: return launch_fast_sel<T, 16, MODE>(a, s);
-    case 32: return launch_fast_sel<T, 32, MODE>(a, s);
-    case 64: return launch_fast_sel<T, 64, MODE>(a, s);
-    case 128: return launch_fast_sel<T, 128, MODE>(a, s);
-    case 256: return launch_fast_sel<T, 256, MODE>(a, s);
-    case 512: return launch_fast_sel<T, 512, MODE>(a, s);
-    case 1024: return launch_fast_sel<T, 1024, MODE>(a, s);
-    case 2048: return launch_fast_sel<T, 2048, MODE>(a, s);
-    case 4096: return launch_fast_sel<T, 4096, MODE>(a, s);
+    case 16: return launch_fast_sel<T, 16, MODE, DK>(a, s);
+    case 32: return launch_fast_sel<T, 32, MODE, DFT_VALU>(a, s); // radices 8, 4
+    case 64: return launch_fast_sel<T, 64, MODE, DK>(a, s);
+    case 128: return launch_fast_sel<T, 128, MODE, DK>(a, s);
+    case 256: return launch_fast_sel<T, 256, MODE, DK>(a, s);
+    case 512: return launch_fast_sel<T, 512, MODE, DK>(a, s);
+    case 1024: return launch_fast_sel<T, 1024, MODE, DK>(a, s);
+    case 2048: return launch_fast_sel<T, 2048, MODE, DK>(a, s);
+    case 4096: return launch_fast_sel<T, 4096, MODE, DK>(a, s);
     default: return hipErrorNotSupported;
     }
 }
 
+template <typename T, int DK>
+static hipError_t launch_fast_dk(uint32_t L, int mode, const FastArgs& a, hipStream_t s) {
+    if (mode == FAST_COL)
+        return launch_fast_mode<T, FAST_COL, DK>(L, a, s);
+    if (mode == FAST_TOUT)
+        return launch_fast_mode<T, FAST_TOUT, DK>(L, a, s);
+    return launch_fast_mode<T, FAST_ROW, DK>(L, a, s);
+}
+
+// KFA_FAST_MFMA_F16 / KFA_FAST_MFMA_BF16 in a.flags select the split-operand
+// radix-16 stages (FP32 only; the FP64 passes always run on the vector units)
 template <typename T>
 static hipError_t launch_fast_any(uint32_t L, int mode, const FastArgs& a, hipStream_t s) {
-    if (mode == FAST_COL)
-        return launch_fast_mode<T, FAST_COL>(L, a, s);
-    if (mode == FAST_TOUT)
-        return launch_fast_mode<T, FAST_TOUT>(L, a, s);
-    return launch_fast_mode<T, FAST_ROW>(L, a, s);
+    if constexpr (std::is_same<T, float>::value) {
+        if (a.flags & KFA_FAST_MFMA_F16)
+            return launch_fast_dk<T, DFT_SPLIT_F16>(L, mode, a, s);
+        if (a.flags & KFA_FAST_MFMA_BF16)
+            return launch_fast_dk<T, DFT_SPLIT_BF16>(L, mode, a, s);
+    }
+    return launch_fast_dk<T, DFT_VALU>(L, mode, a, s);
 }
 
 } // namespace kfa

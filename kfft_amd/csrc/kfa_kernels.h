@@ -43,6 +43,11 @@ hipError_t launch_hadamard_real(int prec, void* acc, const void* other, uint64_t
                                 hipStream_t s);
 hipError_t launch_copy(int prec, const void* in, void* out, uint64_t n, int conj_, double scale,
                        hipStream_t s);
+// refinement step of the precision form: FP32 -> FP64 copy, r = x - z, y += d
+hipError_t launch_widen(const void* in_f32, void* out_f64, uint64_t n, hipStream_t s);
+hipError_t launch_residual(const void* x_f32, const void* z_f64, void* r_f32, uint64_t n,
+                           hipStream_t s);
+hipError_t launch_accumulate(void* y_f32, const void* d_f32, uint64_t n, hipStream_t s);
 hipError_t launch_gather_first(int prec, int real_in, const void* in, int64_t dist_hi,
                                int64_t dist_lo, uint32_t C, void* out, uint64_t batch,
                                hipStream_t s);

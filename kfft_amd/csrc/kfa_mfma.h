@@ -1,0 +1,196 @@
+// Radix-16 DFT stage on the matrix cores (the precision form).
+//
+// A wave holds 64 radix-16 butterflies, one per lane (v[0..15], FP32).  The
+// 16-point DFT of 32 butterflies is one real GEMM
+//     [Re y; Im y] (32 x 32) = [[Fr, -Fi]; [Fi, Fr]] (32 x 32) . [Re x; Im x] (32 x 32)
+// computed with v_mfma_f32_32x32x16_{f16,bf16} (two K=16 halves, FP32 accumulate).
+// Operands are split so that the products keep ~FP32 accuracy:
+//   SPLIT_F16:  x = hi + lo (FP16 each), F likewise; A_hi B_hi + A_hi B_lo + A_lo B_hi.
+//               FP16's range is handled per butterfly: x is scaled by a power of two
+//               that puts max|x| in [2^14, 2^15) and the outputs are scaled back.
+//   SPLIT_BF16: x = hi + mid + lo (BF16 each, 24 bits), F likewise; the six products
+//               down to 2^-24 (hh, hm, mh, hl, lh, mm).
+// Lane layout of the 32x32x16 instruction (verified on gfx950): lane l holds
+// A[l%32][8h+i], B[8h+i][l%32] (h = l/32, i < 8) and D[8(j/4)+4h+(j%4)][l%32].
+// Butterfly b of the wave is column b%32 of group b/32; a lane therefore feeds
+// half of a column (x_{8h..8h+7}) and receives half of its outputs
+// (y_{4h..4h+3}, y_{8+4h..8+4h+3}).  v_permlane32_swap (lanes 32..63 of its
+// first operand trade places with lanes 0..31 of its second) moves the data
+// between the owner layout and the matrix layout: swapping (x_0..7, x_8..15)
+// yields the B operands of both groups at once, and swapping the two groups'
+// results (D0, D1) yields the h = 0 and h = 1 output rows of the lane's own
+// butterfly.
+#pragma once
+#include "kfa_device.h"
+
+namespace kfa {
+
+enum { DFT_VALU = 0, DFT_SPLIT_F16 = 1, DFT_SPLIT_BF16 = 2 };
+
+typedef _Float16 kfa_h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 kfa_b8 __attribute__((ext_vector_type(8)));
+typedef float kfa_f16v __attribute__((ext_vector_type(16)));
+
+template <int SPLIT>
+struct MfmaConst {
+    // A0 multiplies Re x, A1 multiplies Im x; [term] = hi, lo (F16) or hi, mid, lo (BF16)
+    kfa_h8 a0h[2], a1h[2];
+    kfa_b8 a0b[3], a1b[3];
+};
+
+// cos / sin of 2 pi j / 16 (FP64 series, rounded once); lanes index it with
+// their own j, so it lives in memory rather than in registers
+struct Trig16 {
+    float c[16], s[16];
+    constexpr Trig16() : c(), s() {
+        for (int j = 0; j < 16; ++j) {
+            const CS v = cs_turn(j, 16);
+            c[j] = (float)v.c;
+            s[j] = (float)v.s;
+        }
+    }
+};
+__constant__ const Trig16 kfa_trig16{};
+
+template <int SPLIT>
+__device__ __forceinline__ MfmaConst<SPLIT> mfma_dft16_const() {
+    const Trig16& tab = kfa_trig16;
+    MfmaConst<SPLIT> K;
+    const uint32_t l = threadIdx.x & 63u, h = l >> 5, m = l & 31u, mr = m & 15u;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const uint32_t k = 8 * h + i, j = (mr * k) & 15u;
+        const float c = tab.c[j], sn = tab.s[j];
+        // F = exp(-2 pi i m k / 16): Fr = c, Fi = -sn
+        const float a0 = m < 16 ? c : -sn;  // Re y row: Fr ; Im y row: Fi
+        const float a1 = m < 16 ? sn : c;   // Re y row: -Fi ; Im y row: Fr
+        if constexpr (SPLIT == DFT_SPLIT_F16) {
+            const _Float16 h0 = (_Float16)a0, h1 = (_Float16)a1;
+            K.a0h[0][i] = h0;
+            K.a0h[1][i] = (_Float16)(a0 - (float)h0);
+            K.a1h[0][i] = h1;
+            K.a1h[1][i] = (_Float16)(a1 - (float)h1);
+        } else {
+            const __bf16 h0 = (__bf16)a0, h1 = (__bf16)a1;
+            const __bf16 m0 = (__bf16)(a0 - (float)h0), m1 = (__bf16)(a1 - (float)h1);
+            K.a0b[0][i] = h0;
+            K.a0b[1][i] = m0;
+            K.a0b[2][i] = (__bf16)(a0 - (float)h0 - (float)m0);
+            K.a1b[0][i] = h1;
+            K.a1b[1][i] = m1;
+            K.a1b[2][i] = (__bf16)(a1 - (float)h1 - (float)m1);
+        }
+    }
+    return K;
+}
+
+// (a, b) -> (a with lanes 32..63 replaced by b's lanes 0..31,
+//            b with lanes 0..31 replaced by a's lanes 32..63)
+__device__ __forceinline__ void swap32(float& a, float& b) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b),
+                                                    false, false);
+    a = __uint_as_float(r[0]);
+    b = __uint_as_float(r[1]);
+}
+
+// D (32 rows: Re y_0..15, Im y_0..15) for the 32 columns whose half-inputs this
+// lane holds in xr/xi (8 values each)
+template <int SPLIT>
+__device__ __forceinline__ kfa_f16v mfma_dft_group(const MfmaConst<SPLIT>& K, const float* xr,
+                                                   const float* xi) {
+    kfa_f16v acc = {0};
+    if constexpr (SPLIT == DFT_SPLIT_F16) {
+        kfa_h8 rh, rl, ih, il;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            rh[i] = (_Float16)xr[i];
+            rl[i] = (_Float16)(xr[i] - (float)rh[i]);
+            ih[i] = (_Float16)xi[i];
+            il[i] = (_Float16)(xi[i] - (float)ih[i]);
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(K.a0h[1], rh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(K.a0h[0], rl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(K.a1h[1], ih, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(K.a1h[0], il, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(K.a0h[0], rh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(K.a1h[0], ih, acc, 0, 0, 0);
+    } else {
+        kfa_b8 r[3], im[3];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const __bf16 r0 = (__bf16)xr[i], i0 = (__bf16)xi[i];
+            const __bf16 r1 = (__bf16)(xr[i] - (float)r0), i1 = (__bf16)(xi[i] - (float)i0);
+            r[0][i] = r0;
+            r[1][i] = r1;
+            r[2][i] = (__bf16)(xr[i] - (float)r0 - (float)r1);
+            im[0][i] = i0;
+            im[1][i] = i1;
+            im[2][i] = (__bf16)(xi[i] - (float)i0 - (float)i1);
+        }
+        // smallest terms first: mm, lh, hl, mh, hm, hh
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(K.a0b[1], r[1], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(K.a1b[1], im[1], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(K.a0b[2], r[0], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(K.a1b[2], im[0], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(K.a0b[0], r[2], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(K.a1b[0], im[2], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(K.a0b[1], r[0], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(K.a1b[1], im[0], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(K.a0b[0], r[1], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(K.a1b[0], im[1], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(K.a0b[0], r[0], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(K.a1b[0], im[0], acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// In-place forward DFT16 of this lane's butterfly v[0..15]; every lane of the
+// wave must call it together.
+template <int SPLIT>
+__device__ __forceinline__ void dft16_mfma(cx<float>* v, const MfmaConst<SPLIT>& K) {
+    float sc = 1.0f, inv = 1.0f;
+    if constexpr (SPLIT == DFT_SPLIT_F16) {
+        float mx = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            mx = fmaxf(mx, fmaxf(fabsf(v[i].re), fabsf(v[i].im)));
+        if (mx > 0.0f) {
+            int e = ilogbf(mx);
+            e = e > 120 ? 120 : (e < -110 ? -110 : e);
+            sc = ldexpf(1.0f, 14 - e);
+            inv = ldexpf(1.0f, e - 14);
+        }
+    }
+    // owner layout -> matrix layout: (x_i, x_{8+i}) -> (group 0 operand, group 1 operand)
+    float g0r[8], g0i[8], g1r[8], g1i[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        g0r[i] = v[i].re * sc;
+        g0i[i] = v[i].im * sc;
+        g1r[i] = v[8 + i].re * sc;
+        g1i[i] = v[8 + i].im * sc;
+        swap32(g0r[i], g1r[i]);
+        swap32(g0i[i], g1i[i]);
+    }
+    kfa_f16v d0 = mfma_dft_group<SPLIT>(K, g0r, g0i);
+    kfa_f16v d1 = mfma_dft_group<SPLIT>(K, g1r, g1i);
+    // matrix layout -> owner layout: d0 becomes the h = 0 rows, d1 the h = 1 rows
+    float a[16], b[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        a[j] = d0[j];
+        b[j] = d1[j];
+        swap32(a[j], b[j]);
+    }
+    // row j of h: j 0..3 Re y_{4h+j}, 4..7 Re y_{8+4h+j-4}, 8..11 Im y_{4h+j-8},
+    // 12..15 Im y_{8+4h+j-12}
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        v[j] = {a[j] * inv, a[8 + j] * inv};
+        v[8 + j] = {a[4 + j] * inv, a[12 + j] * inv};
+        v[4 + j] = {b[j] * inv, b[8 + j] * inv};
+        v[12 + j] = {b[4 + j] * inv, b[12 + j] * inv};
+    }
+}
+
+} // namespace kfa

@@ -140,3 +140,14 @@ def test_shift2_and_math_host():
     h = a.copy()
     lib.kfft_math_hadamard_cpx(C.c_void_p(h.ctypes.data), C.c_void_p(a.ctypes.data), x * y)
     assert np.allclose(h, a * a, rtol=1e-15)
+
+
+def test_bench_radix16_stage_count():
+    """bench.py's matrix-core work count: radix-16 stages of the shaped passes."""
+    import bench
+
+    assert bench._radix16_stages("direct n=65536 inner=65536 four-step 256x256") == 4
+    assert bench._radix16_stages("direct n=4096 inner=4096 one-pass") == 3
+    assert bench._radix16_stages("direct n=512 inner=512 one-pass") == 1  # 16, 8, 4
+    assert bench._radix16_stages("direct n=32 inner=32 one-pass") == 0    # 8, 4
+    assert bench._radix16_stages("rader n=1009 inner=1008 one-pass") == 0
