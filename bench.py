@@ -158,8 +158,8 @@ def main():
                          "default size) so that profiler byte counters can be calibrated")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the batch shards trivially across "
-                          "GPUs (see DESIGN.md) but multi-GPU runs are not part of it",
+        print(json.dumps({"error": "this benchmark runs on one GPU; the batch shards across GPUs "
+                          "with no exchange, but multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 

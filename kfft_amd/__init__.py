@@ -58,7 +58,10 @@ EXPORTS = [
     "kfft_scalar_out_len", "kfft_eval_scalar_batch", "kfft_evali_scalar_batch",
     "kfft_eval_scalar_batch_async", "kfft_evali_scalar_batch_async",
     "kfft_eval2_cpx_async",
+    "kfft_config_ragged", "kfft_ragged_layout", "kfft_eval_ragged", "kfft_eval_ragged_async",
+    "kfft_ragged_error", "kfft_ragged_describe", "kfft_release_ragged",
 ]
+RAGGED_COMPLEX, RAGGED_REAL = 0, 1
 
 
 class KfftError(RuntimeError):
@@ -137,6 +140,13 @@ def load(half: bool = False):
         "kfft_eval_scalar_batch_async": (ret, [vp, vp, vp, u32, vp]),
         "kfft_evali_scalar_batch_async": (ret, [vp, vp, vp, u32, vp]),
         "kfft_eval2_cpx_async": (ret, [vp, vp, vp, vp]),
+        "kfft_config_ragged": (vp, [u32, vp, vp, vp, vp, u32]),
+        "kfft_ragged_layout": (ret, [vp, vp, vp, vp, vp]),
+        "kfft_eval_ragged": (ret, [vp, vp, vp]),
+        "kfft_eval_ragged_async": (ret, [vp, vp, vp, vp]),
+        "kfft_ragged_error": (C.c_int, [vp]),
+        "kfft_ragged_describe": (C.c_char_p, [vp]),
+        "kfft_release_ragged": (None, [vp]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -368,6 +378,59 @@ class Conv2Plan(_Plan):
         ev = self.L.kfft_eval2_conv_scalar if self.real else self.L.kfft_eval2_conv_cpx
         check(ev(self._p, _ptr(a), _ptr(b), _ptr(out)), "kfft_eval2_conv", self.half)
         return out
+
+
+class RaggedPlan:
+    """kfft_config_ragged / kfft_eval_ragged: uneven lengths in one persistent launch.
+
+    `lengths` and `kinds` (RAGGED_COMPLEX / RAGGED_REAL) per transform; byte
+    offsets default to the packed layout.  Buffers are device tensors of bytes
+    (torch.uint8) or any device tensor spanning in_bytes / out_bytes."""
+
+    def __init__(self, lengths, kinds=None, flags: int = 0, half: bool = False,
+                 in_off=None, out_off=None):
+        self.half = half
+        self.L = load(half)
+        self.count = len(lengths)
+        self.lengths = np.ascontiguousarray(lengths, dtype=np.uint32)
+        self.kinds = np.ascontiguousarray(
+            kinds if kinds is not None else np.zeros(self.count), dtype=np.uint8)
+        io = np.ascontiguousarray(in_off, dtype=np.uint64) if in_off is not None else None
+        oo = np.ascontiguousarray(out_off, dtype=np.uint64) if out_off is not None else None
+        self._p = self.L.kfft_config_ragged(
+            self.count, _ptr(self.lengths), _ptr(self.kinds),
+            _ptr(io) if io is not None else None, _ptr(oo) if oo is not None else None, flags)
+        if not self._p:
+            raise KfftError(f"kfft_config_ragged failed: {last_error(half)}")
+        self.in_off = np.empty(self.count, dtype=np.uint64)
+        self.out_off = np.empty(self.count, dtype=np.uint64)
+        ib, ob = C.c_uint64(0), C.c_uint64(0)
+        check(self.L.kfft_ragged_layout(self._p, _ptr(self.in_off), _ptr(self.out_off),
+                                        C.byref(ib), C.byref(ob)), "kfft_ragged_layout", half)
+        self.in_bytes, self.out_bytes = ib.value, ob.value
+
+    def __del__(self):
+        try:
+            if self._p:
+                self.L.kfft_release_ragged(self._p)
+                self._p = None
+        except Exception:
+            pass
+
+    def describe(self) -> str:
+        return self.L.kfft_ragged_describe(self._p).decode()
+
+    def __call__(self, inp, out):
+        check(self.L.kfft_eval_ragged(self._p, _ptr(inp), _ptr(out)), "kfft_eval_ragged",
+              self.half)
+        return out
+
+    def run_async(self, inp, out, stream=None):
+        check(self.L.kfft_eval_ragged_async(self._p, _ptr(inp), _ptr(out), stream),
+              "kfft_eval_ragged_async", self.half)
+
+    def error(self) -> int:
+        return self.L.kfft_ragged_error(self._p)
 
 
 def synchronize(half: bool = False) -> None:

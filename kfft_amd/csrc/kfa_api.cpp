@@ -1438,4 +1438,109 @@ KFFT_API kfft_return_t kfft_eval2_cpx_async(kfft_plan_c2d* plan, const kfft_cpx*
     return c2d_run_dev(plan, fin, fout, s);
 }
 
+// ----------------------------------------------------------- ragged batch
+struct kfft_plan_ragged {
+    std::unique_ptr<RaggedPlan> plan;
+    std::vector<uint64_t> in_off, out_off;
+    uint64_t in_bytes = 0, out_bytes = 0;
+};
+
+// executor shape, from the environment where set: KFFT_AMD_RAGGED_IPW (largest
+// instances per tile), KFFT_AMD_RAGGED_WGS (workgroups per CU),
+// KFFT_AMD_RAGGED_LDS (KiB of LDS per workgroup for tiles), KFFT_AMD_RAGGED_GRAPH
+// (0: launch directly instead of replaying a captured graph)
+static RaggedOptions ragged_options() {
+    RaggedOptions o;
+    if (const char* e = getenv("KFFT_AMD_RAGGED_IPW"))
+        o.ipw_cap = (uint32_t)atoi(e);
+    if (const char* e = getenv("KFFT_AMD_RAGGED_WGS"))
+        o.wgs_per_cu = (uint32_t)atoi(e);
+    if (const char* e = getenv("KFFT_AMD_RAGGED_LDS"))
+        o.lds_budget = (uint32_t)atoi(e) * 1024u;
+    if (const char* e = getenv("KFFT_AMD_RAGGED_GRAPH"))
+        o.graph = atoi(e) != 0;
+    return o;
+}
+
+KFFT_API kfft_plan_ragged* kfft_config_ragged(uint32_t count, const uint32_t* nfft,
+                                              const uint8_t* kind, const uint64_t* in_off,
+                                              const uint64_t* out_off, uint32_t flags) {
+    if (!count || !nfft || !runtime_init())
+        return nullptr;
+    std::unique_ptr<kfft_plan_ragged> R(new (std::nothrow) kfft_plan_ragged());
+    if (!R)
+        return nullptr;
+    std::vector<RaggedItem> items(count);
+    uint64_t ip = 0, op = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        RaggedItem& it = items[i];
+        it.n = nfft[i];
+        it.kind = kind ? kind[i] : KFFT_RAGGED_COMPLEX;
+        const uint64_t in_b = (uint64_t)it.n * (it.kind == KFFT_RAGGED_REAL ? sizeof(kfft_scalar)
+                                                                            : CPXB);
+        const uint64_t outn = it.kind == KFFT_RAGGED_REAL && it.n % 2 == 0 ? it.n / 2 + 1 : it.n;
+        const uint64_t out_b = outn * CPXB;
+        it.in_off = in_off ? in_off[i] : ip;
+        it.out_off = out_off ? out_off[i] : op;
+        ip = (it.in_off + in_b + 255) / 256 * 256;
+        op = (it.out_off + out_b + 255) / 256 * 256;
+        R->in_bytes = std::max(R->in_bytes, it.in_off + in_b);
+        R->out_bytes = std::max(R->out_bytes, it.out_off + out_b);
+        R->in_off.push_back(it.in_off);
+        R->out_off.push_back(it.out_off);
+    }
+    const bool inverse = (flags & KFFT_FLAG_INVERSE) != 0;
+    R->plan.reset(new (std::nothrow)
+                      RaggedPlan(PREC, inverse, !(flags & KFFT_FLAG_DISABLE_NORM)));
+    if (!R->plan || !R->plan->build(items, ragged_options()))
+        return nullptr;
+    return R.release();
+}
+
+KFFT_API kfft_return_t kfft_ragged_layout(const kfft_plan_ragged* plan, uint64_t* in_off,
+                                          uint64_t* out_off, uint64_t* in_bytes,
+                                          uint64_t* out_bytes) {
+    if (!plan)
+        return KFFT_RET_BADARGUMENTS;
+    if (in_off)
+        memcpy(in_off, plan->in_off.data(), plan->in_off.size() * 8);
+    if (out_off)
+        memcpy(out_off, plan->out_off.data(), plan->out_off.size() * 8);
+    if (in_bytes)
+        *in_bytes = plan->in_bytes;
+    if (out_bytes)
+        *out_bytes = plan->out_bytes;
+    return KFFT_RET_SUCCESS;
+}
+
+KFFT_API kfft_return_t kfft_eval_ragged_async(kfft_plan_ragged* plan, const void* in, void* out,
+                                              void* stream) {
+    if (!plan || !in || !out || !is_device_ptr(in) || !is_device_ptr(out))
+        return KFFT_RET_BADARGUMENTS;
+    hipStream_t s = stream ? (hipStream_t)stream : default_stream();
+    return hip_ok(plan->plan->run(in, out, s), "ragged run") ? KFFT_RET_SUCCESS
+                                                             : KFFT_RET_GPU_FAIL;
+}
+
+KFFT_API kfft_return_t kfft_eval_ragged(kfft_plan_ragged* plan, const void* in, void* out) {
+    if (!plan || !in || !out || !is_device_ptr(in) || !is_device_ptr(out))
+        return KFFT_RET_BADARGUMENTS;
+    hipStream_t s = entry_stream();
+    if (!hip_ok(plan->plan->run(in, out, s), "ragged run"))
+        return KFFT_RET_GPU_FAIL;
+    if (plan->plan->error()) {
+        set_error("ragged batch: a dependency wait timed out");
+        return KFFT_RET_GPU_FAIL;
+    }
+    return KFFT_RET_SUCCESS;
+}
+
+KFFT_API int kfft_ragged_error(kfft_plan_ragged* plan) { return plan ? plan->plan->error() : 0; }
+
+KFFT_API const char* kfft_ragged_describe(const kfft_plan_ragged* plan) {
+    return plan ? plan->plan->describe() : "";
+}
+
+KFFT_API void kfft_release_ragged(kfft_plan_ragged* plan) { delete plan; }
+
 } // extern "C"

@@ -1,7 +1,9 @@
 // Auxiliary gfx950 kernels of the transform engine: exact twiddle tables,
 // Bluestein/Rader precompute, real-FFT pre/post passes, tiled transposes,
 // pointwise products and copies; plus the host launchers.
-#include "kfa_device.h"
+#include <cstring>
+
+#include "kfa_aux_dev.h"
 #include "kfa_kernels.h"
 
 namespace kfa {
@@ -105,48 +107,6 @@ __global__ void k_convert_from_f64(cx<T>* dst, const cx<double>* src, uint64_t n
         dst[i] = {(T)(src[i].re * scale), (T)(src[i].im * scale)};
 }
 
-// Real-input post-processing (even length n = 2h): X[k] from Z = FFT_h(x as complex)
-//   X[k]   = ((f1 + t)/2),  X[h-k] = conj((f1 - t)/2)... written out per component
-// with f1 = Z[k] + conj(Z[h-k]), f2 = Z[k] - conj(Z[h-k]), t = f2 * st[k-1].
-// Optional full-spectrum mirror X[n-k] = conj(X[k]).  General strided layout.
-template <typename T>
-__global__ void k_real_post(const cx<T>* Z, int64_t z_dist, cx<T>* X, int64_t x_dist_hi,
-                            int64_t x_dist_lo, int64_t x_stride, uint32_t C, uint32_t h,
-                            uint64_t batch, const cx<T>* st, int expand, int conj_out) {
-    const uint32_t kk = blockIdx.x * blockDim.x + threadIdx.x; // 0..h/2
-    const uint64_t b = blockIdx.y + (uint64_t)gridDim.y * blockIdx.z;
-    if (b >= batch || kk > h / 2)
-        return;
-    const cx<T>* z = Z + (int64_t)b * z_dist;
-    const uint64_t bh = b / C, bl = b - bh * C;
-    cx<T>* x = X + (int64_t)bh * x_dist_hi + (int64_t)bl * x_dist_lo;
-    const uint32_t n = 2 * h;
-    auto put = [&](uint32_t k, cx<T> v) {
-        if (conj_out)
-            v.im = -v.im;
-        x[(int64_t)k * x_stride] = v;
-    };
-    if (kk == 0) {
-        const cx<T> dc = z[0];
-        put(0, {dc.re + dc.im, T(0)});
-        put(h, {dc.re - dc.im, T(0)});
-        return;
-    }
-    const uint32_t k = kk;
-    const cx<T> a = z[k], bz = z[h - k];
-    const cx<T> bb = {bz.re, -bz.im};
-    const cx<T> f1 = a + bb, f2 = a - bb;
-    const cx<T> t = cmul(f2, st[k - 1]);
-    const cx<T> xk = {(f1.re + t.re) * T(0.5), (f1.im + t.im) * T(0.5)};
-    const cx<T> xh = {(f1.re - t.re) * T(0.5), (t.im - f1.im) * T(0.5)};
-    put(k, xk);
-    put(h - k, xh);
-    if (expand) {
-        put(n - k, conj(xk));
-        put(n - (h - k), conj(xh));
-    }
-}
-
 // Real-output pre-processing (even n = 2h): Z[k] from the half spectrum X[0..h]
 // (the reference's rebuild_inverse, which yields 2*Z).  General strided input.
 template <typename T>
@@ -248,42 +208,26 @@ __global__ void k_accumulate(cx<float>* y, const cx<float>* d, uint64_t n) {
     }
 }
 
-// out[b] = x_b[0] for every instance b (the Rader x[0] values), so that passes
-// writing the output in place never race with readers of x[0].
 template <typename T>
-__global__ void k_gather_first(const T* in, int64_t dist_hi, int64_t dist_lo, uint32_t C, T* out,
-                               uint64_t batch, int comps) {
-    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= batch)
-        return;
-    const uint64_t hi = b / C, lo = b - hi * C;
-    const int64_t off = (int64_t)hi * dist_hi + (int64_t)lo * dist_lo;
-    for (int c = 0; c < comps; ++c)
-        out[b * comps + c] = in[off * comps + c];
+__global__ void k_real_post(const RealPostArgs a) {
+    const uint32_t kk = blockIdx.x * blockDim.x + threadIdx.x; // 0..h/2
+    const uint64_t b = blockIdx.y + (uint64_t)gridDim.y * blockIdx.z;
+    if (b < a.batch && kk <= a.h / 2)
+        real_post_one<T>(a, kk, b);
 }
 
-// Rader DC term: X[0] = scale * (x[0] + conj?(A[0])), A = FFT_{p-1} of the
-// gathered input (kept in scratch), written through the output layout.
 template <typename T>
-__global__ void k_rader_dc(const T* x0, int x0_real, const cx<T>* A, uint32_t L, T* out,
-                           int64_t dh, int64_t dl, uint32_t C, int real_out, int conj_,
-                           double scale, uint64_t batch) {
+__global__ void k_gather_first(const GatherArgs a) {
     const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= batch)
-        return;
-    cx<T> a = A[b * L];
-    if (conj_)
-        a.im = -a.im;
-    const cx<T> xv = x0_real ? cx<T>{x0[b], T(0)} : cx<T>{x0[2 * b], x0[2 * b + 1]};
-    cx<T> d = xv + a;
-    d.re *= (T)scale;
-    d.im *= (T)scale;
-    const uint64_t hi = b / C, lo = b - hi * C;
-    const int64_t off = (int64_t)hi * dh + (int64_t)lo * dl;
-    if (real_out)
-        out[off] = d.re;
-    else
-        ((cx<T>*)out)[off] = d;
+    if (b < a.batch)
+        gather_first_one<T>(a, b);
+}
+
+template <typename T>
+__global__ void k_rader_dc(const DcArgs a) {
+    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < a.batch)
+        rader_dc_one<T>(a, b);
 }
 
 // ---------------------------------------------------------------------------
@@ -292,6 +236,8 @@ __global__ void k_rader_dc(const T* x0, int x0_real, const cx<T>* A, uint32_t L,
 static inline unsigned blocks_for(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
 hipError_t launch_fft(int prec, int maxe, const KfaFftArgs& a, uint32_t lds_bytes, hipStream_t s) {
+    if (Recorder* r = recorder())
+        return r->add(KFA_REC_FFT, prec, maxe, lds_bytes, 0, 0, &a, sizeof(a));
     return prec == 8 ? launch_fft_f64(maxe, a, lds_bytes, s) : launch_fft_f32(maxe, a, lds_bytes, s);
 }
 
@@ -365,19 +311,35 @@ static dim3 batch_grid(uint32_t gx, uint64_t batch) {
 
 hipError_t launch_real_post(int prec, const void* Z, int64_t z_dist, void* X, int64_t xdh,
                             int64_t xdl, int64_t xs, uint32_t C, uint32_t h, uint64_t batch,
-                            const void* st, int expand, int conj_out, hipStream_t s) {
+                            const void* st, int expand, int conj_out, hipStream_t s,
+                            const int64_t* ztbl, const int64_t* xtbl) {
     if (!batch)
         return hipSuccess;
+    RealPostArgs a;
+    memset(&a, 0, sizeof(a));
+    a.Z = (uint64_t)Z;
+    a.X = (uint64_t)X;
+    a.st = (uint64_t)st;
+    a.ztbl = (uint64_t)ztbl;
+    a.xtbl = (uint64_t)xtbl;
+    a.z_dist = z_dist;
+    a.xdh = xdh;
+    a.xdl = xdl;
+    a.xs = xs;
+    a.batch = batch;
+    a.C = C;
+    a.h = h;
+    a.expand = (uint32_t)expand;
+    a.conj_out = (uint32_t)conj_out;
     const unsigned bs = 128;
+    if (Recorder* r = recorder())
+        return r->add(KFA_REC_REAL_POST, prec, 0, 0, blocks_for(h / 2 + 1, KFA_RAG_THREADS) * batch,
+                      blocks_for(h / 2 + 1, KFA_RAG_THREADS), &a, sizeof(a));
     const dim3 g = batch_grid(blocks_for(h / 2 + 1, bs), batch);
     if (prec == 8)
-        hipLaunchKernelGGL(k_real_post<double>, g, dim3(bs), 0, s, (const cx<double>*)Z, z_dist,
-                           (cx<double>*)X, xdh, xdl, xs, C, h, batch, (const cx<double>*)st,
-                           expand, conj_out);
+        hipLaunchKernelGGL(k_real_post<double>, g, dim3(bs), 0, s, a);
     else
-        hipLaunchKernelGGL(k_real_post<float>, g, dim3(bs), 0, s, (const cx<float>*)Z, z_dist,
-                           (cx<float>*)X, xdh, xdl, xs, C, h, batch, (const cx<float>*)st, expand,
-                           conj_out);
+        hipLaunchKernelGGL(k_real_post<float>, g, dim3(bs), 0, s, a);
     return hipGetLastError();
 }
 
@@ -483,32 +445,55 @@ hipError_t launch_accumulate(void* y, const void* d, uint64_t n, hipStream_t s) 
 
 hipError_t launch_gather_first(int prec, int real_in, const void* in, int64_t dist_hi,
                                int64_t dist_lo, uint32_t C, void* out, uint64_t batch,
-                               hipStream_t s) {
+                               hipStream_t s, const int64_t* tbl) {
     if (!batch)
         return hipSuccess;
-    const int comps = real_in ? 1 : 2;
+    GatherArgs a;
+    memset(&a, 0, sizeof(a));
+    a.in = (uint64_t)in;
+    a.out = (uint64_t)out;
+    a.tbl = (uint64_t)tbl;
+    a.dist_hi = dist_hi;
+    a.dist_lo = dist_lo;
+    a.batch = batch;
+    a.C = C;
+    a.comps = real_in ? 1 : 2;
+    if (Recorder* r = recorder())
+        return r->add(KFA_REC_GATHER, prec, 0, 0, blocks_for(batch, KFA_RAG_THREADS), 0, &a,
+                      sizeof(a));
     if (prec == 8)
-        hipLaunchKernelGGL(k_gather_first<double>, dim3(blocks_for(batch, 256)), dim3(256), 0, s,
-                           (const double*)in, dist_hi, dist_lo, C, (double*)out, batch, comps);
+        hipLaunchKernelGGL(k_gather_first<double>, dim3(blocks_for(batch, 256)), dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL(k_gather_first<float>, dim3(blocks_for(batch, 256)), dim3(256), 0, s,
-                           (const float*)in, dist_hi, dist_lo, C, (float*)out, batch, comps);
+        hipLaunchKernelGGL(k_gather_first<float>, dim3(blocks_for(batch, 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_rader_dc(int prec, const void* x0, int x0_real, const void* A, uint32_t L,
                            void* out, int64_t dh, int64_t dl, uint32_t C, int real_out, int conj_,
-                           double scale, uint64_t batch, hipStream_t s) {
+                           double scale, uint64_t batch, hipStream_t s, const int64_t* tbl) {
     if (!batch)
         return hipSuccess;
+    DcArgs a;
+    memset(&a, 0, sizeof(a));
+    a.x0 = (uint64_t)x0;
+    a.A = (uint64_t)A;
+    a.out = (uint64_t)out;
+    a.tbl = (uint64_t)tbl;
+    a.dh = dh;
+    a.dl = dl;
+    a.scale = scale;
+    a.batch = batch;
+    a.L = L;
+    a.C = C;
+    a.x0_real = (uint32_t)x0_real;
+    a.real_out = (uint32_t)real_out;
+    a.conj_ = (uint32_t)conj_;
+    if (Recorder* r = recorder())
+        return r->add(KFA_REC_DC, prec, 0, 0, blocks_for(batch, KFA_RAG_THREADS), 0, &a, sizeof(a));
     if (prec == 8)
-        hipLaunchKernelGGL(k_rader_dc<double>, dim3(blocks_for(batch, 256)), dim3(256), 0, s,
-                           (const double*)x0, x0_real, (const cx<double>*)A, L, (double*)out, dh,
-                           dl, C, real_out, conj_, scale, batch);
+        hipLaunchKernelGGL(k_rader_dc<double>, dim3(blocks_for(batch, 256)), dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL(k_rader_dc<float>, dim3(blocks_for(batch, 256)), dim3(256), 0, s,
-                           (const float*)x0, x0_real, (const cx<float>*)A, L, (float*)out, dh, dl,
-                           C, real_out, conj_, scale, batch);
+        hipLaunchKernelGGL(k_rader_dc<float>, dim3(blocks_for(batch, 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -27,12 +27,14 @@
 #define KFA_IO_PW_F64 0x1000u     // pw table is double precision complex
 #define KFA_IO_SCALE_POST 0x2000u // multiply by KfaFftArgs::scale_post after ADDX0
 #define KFA_IO_AUX_REAL 0x4000u   // aux (x[0] source) is a real array
+#define KFA_IO_INST 0x8000u       // instance bases from a table: base = inst[b_hi] + b_lo*dist_lo
 
 // Addressing of element e of instance b on one side of a pass:
 //   b_lo = b % C, b_hi = b / C
 //   lg   = b_lo * lo_mul + e * e_mul              (logical index)
 //   ph   = (flags & MAP) ? map[lg] : lg           (physical index)
 //   addr = b_hi * dist_hi + b_lo * dist_lo + ph * stride   (in elements)
+//   (KFA_IO_INST: inst[b_hi] replaces b_hi * dist_hi; ragged batches)
 // lg >= valid loads zero / skips the store.
 struct KfaIo {
     uint64_t ptr;
@@ -41,6 +43,7 @@ struct KfaIo {
     uint64_t pw;
     uint64_t aux; // ADDX0: x[0] values, indexed by b_hi*aux_dist + b_lo*aux_dist_lo
     int64_t aux_dist, aux_dist_lo;
+    uint64_t inst; // KFA_IO_INST: int64 element offset of every instance
     uint32_t C, lo_mul, e_mul, valid;
     uint32_t flags, pad0;
 };
@@ -94,6 +97,51 @@ struct FastArgs {
     double scale;
 };
 
+
+// Arguments of the per-instance helper kernels (kfa_aux_dev.h); a non-zero
+// table replaces the instance distance (ragged batches).
+struct GatherArgs { // out[b] = x_b[0]
+    uint64_t in, out, tbl;
+    int64_t dist_hi, dist_lo;
+    uint64_t batch;
+    uint32_t C, comps;
+};
+struct DcArgs { // Rader X[0] = scale * (x0 + conj?(A[0]))
+    uint64_t x0, A, out, tbl;
+    int64_t dh, dl;
+    double scale;
+    uint64_t batch;
+    uint32_t L, C, x0_real, real_out, conj_, pad;
+};
+struct RealPostArgs { // real-input rebuild, X from Z = FFT_h(x as complex)
+    uint64_t Z, X, st, ztbl, xtbl;
+    int64_t z_dist, xdh, xdl, xs;
+    uint64_t batch;
+    uint32_t C, h, expand, conj_out;
+};
+
+// Ragged executor (kfa_ragged.hip): one recorded launch of a length group.
+#define KFA_REC_FFT 1u
+#define KFA_REC_GATHER 2u
+#define KFA_REC_DC 3u
+#define KFA_REC_REAL_POST 4u
+struct RagRecord {
+    uint32_t type;      // KFA_REC_*
+    uint32_t tiles;     // workgroup tiles of this launch
+    uint32_t gx;        // REAL_POST: tiles along k (tile = b * gx + kx)
+    int32_t dep;        // record that must be complete first (same queue), -1 none
+    uint32_t dep_tiles; // its tile count
+    uint32_t pad;
+    uint64_t args;      // byte offset of the launch arguments in the blob
+};
+struct RagArgs {
+    const RagRecord* rec;
+    const unsigned char* blob;
+    const uint32_t* tq;     // tickets: (record, tile) pairs, queue by queue
+    const uint32_t* tq_off; // first ticket of each queue (nq + 1 entries)
+    uint32_t* ctr;          // queue tickets (16 words apart), then done[records], then error
+    uint32_t nq, nrec, spin_limit, pad;
+};
 
 // Arguments of the fused on-chip two-pass transform (kfa_fused_impl.h).
 struct FusedArgs {

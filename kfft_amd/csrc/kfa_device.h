@@ -225,7 +225,8 @@ __device__ __forceinline__ Ctx make_ctx(const KfaIo& io, uint64_t inst) {
         hi = inst / io.C;
         lo = (uint32_t)(inst - hi * io.C);
     }
-    c.base = (int64_t)hi * io.dist_hi + (int64_t)lo * io.dist_lo;
+    c.base = ((io.flags & KFA_IO_INST) ? ((const int64_t*)io.inst)[hi] : (int64_t)hi * io.dist_hi) +
+             (int64_t)lo * io.dist_lo;
     c.lo = lo;
     c.lgb = lo * io.lo_mul;
     c.hi = hi;

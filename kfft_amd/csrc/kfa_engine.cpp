@@ -317,7 +317,29 @@ static bool fast_io_ok(const KfaIo& io, uint32_t allowed) {
 }
 
 static hipError_t launch_fast(int prec, uint32_t L, int mode, const FastArgs& a, hipStream_t s) {
+    if (recorder())
+        return hipErrorNotSupported; // shaped passes are not tasks of the ragged executor
     return prec == 8 ? launch_fast_f64(L, mode, a, s) : launch_fast_f32(L, mode, a, s);
+}
+
+// ============================================================== launch recorder
+Recorder*& recorder() {
+    thread_local Recorder* r = nullptr;
+    return r;
+}
+
+hipError_t Recorder::add(uint32_t type, int prec, int maxe, uint32_t lds, uint64_t tiles,
+                         uint32_t gx, const void* args, size_t bytes) {
+    RecOp op;
+    op.type = type;
+    op.prec = (uint32_t)prec;
+    op.maxe = (uint32_t)maxe;
+    op.lds = lds;
+    op.tiles = tiles;
+    op.gx = gx;
+    op.args.assign((const unsigned char*)args, (const unsigned char*)args + bytes);
+    ops.push_back(std::move(op));
+    return hipSuccess;
 }
 
 static FastArgs fast_args(const void* tw, uint64_t batch, double scale) {
@@ -412,11 +434,18 @@ static uint32_t env_u32(const char* name, uint32_t dflt) {
 
 static uint32_t g_num_cus = 0;
 
-uint32_t fused_queues() {
+uint32_t num_cus() {
+    xcd_queues();
+    return g_num_cus;
+}
+
+uint32_t fused_queues() { return env_u32("KFFT_AMD_FUSED", 1) ? xcd_queues() : 0u; }
+
+uint32_t xcd_queues() {
     static std::once_flag once;
     static uint32_t nq = 0;
     std::call_once(once, [] {
-        if (env_u32("KFFT_AMD_FUSED", 1) == 0 || !runtime_init())
+        if (!runtime_init())
             return;
         int dev = 0, cus = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
@@ -505,8 +534,9 @@ hipError_t SubFft::run(const KfaIo& load, const KfaIo& store, double scale, doub
                        uint32_t dft) const {
     const bool simple = fast_io_ok(load, KFA_IO_CONJ1) &&
                         fast_io_ok(store, KFA_IO_CONJ2 | KFA_IO_SCALE) && scale_post == 1.0;
+    const bool rec = recorder() != nullptr; // recording for the ragged executor: general passes
     if (!four) {
-        if (simple && fast_len(L)) {
+        if (simple && fast_len(L) && !rec) {
             const bool rows = load.C == 1 && store.C == 1 && load.stride == 1 && store.stride == 1;
             const bool cols = load.C == store.C && load.dist_lo == 1 && store.dist_lo == 1 &&
                               load.stride != 1 && store.stride != 1;
@@ -532,10 +562,10 @@ hipError_t SubFft::run(const KfaIo& load, const KfaIo& store, double scale, doub
     }
     if (load.C != 1 || store.C != 1)
         return hipErrorInvalidValue;
-    if (onchip && !dft && simple && tw4_full.p && N1 == 256 && N2 == 256 && s == default_stream() &&
+    if (onchip && !rec && !dft && simple && tw4_full.p && N1 == 256 && N2 == 256 && s == default_stream() &&
         load.stride == 1 && store.stride == 1 && fused_queues() > 0)
         return run_fused(load, store, scale, batch, s);
-    if (simple && tw4_full.p) {
+    if (simple && tw4_full.p && !rec) {
         const Tuning& tn = tuning();
         uint64_t chunk = batch;
         if (tn.chunk_bytes) {
@@ -665,6 +695,10 @@ static KfaIo io_from(const Layout& l) {
     io.e_mul = 1;
     io.valid = 0xffffffffu;
     io.flags = l.flags & (KFA_IO_REAL | KFA_IO_REAL_PAIRS);
+    if (l.inst) {
+        io.inst = (uint64_t)l.inst;
+        io.flags |= KFA_IO_INST;
+    }
     return io;
 }
 
@@ -830,7 +864,7 @@ hipError_t Plan1D::execute(const Layout& in, const Layout& out, uint64_t batch, 
         sp += (size_t)batch * L * eb;
         void* sub_scr = sp;
         hipError_t e = launch_gather_first(prec_, real_in, in.ptr, in.dist_hi, in.dist_lo,
-                                           in.C ? in.C : 1, x0, batch, s);
+                                           in.C ? in.C : 1, x0, batch, s, in.inst);
         if (e != hipSuccess)
             return e;
         // pass 1: a[n] = x'[g^n], A = FFT_{p-1}(a)
@@ -858,7 +892,7 @@ hipError_t Plan1D::execute(const Layout& in, const Layout& out, uint64_t batch, 
             return e;
         // X[0] = x0 + A[0]
         return launch_rader_dc(prec_, x0, real_in, tmpA, L, out.ptr, out.dist_hi, out.dist_lo,
-                               out.C ? out.C : 1, real_out, inverse, scale, batch, s);
+                               out.C ? out.C : 1, real_out, inverse, scale, batch, s, out.inst);
     }
     // Bluestein
     void* tmp = sp;

@@ -55,6 +55,7 @@ struct Layout {
     int64_t dist_hi = 0, dist_lo = 0, stride = 1;
     uint32_t C = 1;
     uint32_t flags = 0; // KFA_IO_REAL / KFA_IO_REAL_PAIRS
+    const int64_t* inst = nullptr; // device table of instance offsets (replaces dist_hi)
 };
 
 // Extra operations a caller fuses into a transform (used by the real-FFT,
@@ -118,8 +119,12 @@ struct SubFft {
     mutable bool fused_used = false;
 };
 
-// queues of the fused path (the number of XCDs), 0 when it is unavailable
+// XCDs that host workgroups of an ordinary grid (probed once with XCC_ID), 0
+// when the mapping cannot be established; the fused path's queue count (0 when
+// KFFT_AMD_FUSED=0); compute units of the device
+uint32_t xcd_queues();
 uint32_t fused_queues();
+uint32_t num_cus();
 
 bool subfft_capable(uint32_t L, int prec);
 
@@ -163,5 +168,59 @@ std::shared_ptr<Plan1D> get_plan(uint32_t n, int prec);
 bool make_real_twiddles(DevBuf& out, uint32_t h, int prec, bool inverse);
 
 size_t elem_bytes(int prec); // bytes of one complex value
+
+// ------------------------------------------------------------ ragged executor
+// A batch of transforms of uneven lengths run as one persistent launch
+// (kfa_ragged.hip).  Item kinds: complex (forward or inverse, n outputs) and
+// real-input forward (n/2+1 outputs for even n, n for odd n).  Offsets are in
+// bytes from the base pointers given to run().
+enum { RAGGED_COMPLEX = 0, RAGGED_REAL = 1 };
+
+struct RaggedItem {
+    uint32_t n = 0, kind = RAGGED_COMPLEX;
+    uint64_t in_off = 0, out_off = 0;
+};
+
+struct RaggedOptions {
+    uint32_t lds_budget = 80 * 1024; // per workgroup, for choosing instances per tile
+    uint32_t ipw_cap = 0;            // largest instances per tile (0: as many as fit)
+    uint32_t wgs_per_cu = 0;         // resident workgroups per CU (0: occupancy limit)
+    uint32_t parts_per_queue = 4;    // split of the work of a queue into groups (balance)
+    bool graph = true;               // replay the launch sequence as a captured hipGraph
+};
+
+class RaggedPlan {
+  public:
+    RaggedPlan(int prec, bool inverse, bool norm) : prec_(prec), inverse_(inverse), norm_(norm) {}
+    ~RaggedPlan();
+    bool build(const std::vector<RaggedItem>& items, const RaggedOptions& opt);
+    // asynchronous on s; in / out are device pointers
+    hipError_t run(const void* in, void* out, hipStream_t s);
+    // non-zero if the last run's dependency waits timed out (synchronises)
+    int error();
+    const char* describe() const { return desc_.c_str(); }
+    size_t records() const { return nrec_; }
+    size_t tickets() const { return ntick_; }
+
+  private:
+    int prec_;
+    bool inverse_, norm_;
+    RaggedOptions opt_;
+    std::vector<std::shared_ptr<Plan1D>> plans_;
+    std::vector<std::unique_ptr<DevBuf>> tables_; // real-FFT super twiddles
+    DevBuf tbl_, scratch_, rec_, blob_, tq_, tq_off_, ctr_, patch_off_, patch_which_, patch_delta_;
+    uint32_t nq_ = 0, nrec_ = 0, npatch_ = 0, lds_ = 0, groups_ = 0;
+    size_t ntick_ = 0, ctr_words_ = 0;
+    RagArgs args_ = {};
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+    hipStream_t cap_ = nullptr;  // capture stream (private, so no other work is captured)
+    hipStream_t last_ = nullptr; // stream of the last run
+    const void* g_in_ = nullptr;
+    void* g_out_ = nullptr;
+    std::string desc_;
+    std::mutex mu_;
+    hipError_t enqueue(const void* in, void* out, hipStream_t s);
+};
 
 } // namespace kfa

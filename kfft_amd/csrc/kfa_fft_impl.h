@@ -66,12 +66,12 @@ __device__ __forceinline__ void lds_stage(const KfaFftArgs& A, uint32_t s, cx<T>
             lds_stage<T, R, MAXE>(A, s, my, lane, act);                                           \
         break;
 
+// One workgroup tile of a pass: instances [first, first + ipw).  The
+// workgroup may be larger than tpi * ipw (threads beyond it idle in the
+// stages); LDS must hold ipw * inst_stride elements.
 template <typename T, int MAXE>
-__global__ void __launch_bounds__(MAXE <= 16 ? 512 : 256) k_fft(const KfaFftArgs A) {
-    extern __shared__ __align__(16) unsigned char kfa_smem[];
-    cx<T>* lds = reinterpret_cast<cx<T>*>(kfa_smem);
+__device__ __forceinline__ void fft_tile(const KfaFftArgs& A, uint64_t first, cx<T>* lds) {
     const uint32_t tid = threadIdx.x, nthr = blockDim.x;
-    const uint64_t first = (uint64_t)blockIdx.x * A.ipw;
     const uint32_t nin =
         (uint32_t)((A.batch - first) < (uint64_t)A.ipw ? (A.batch - first) : (uint64_t)A.ipw);
     const uint32_t L = A.L, total = nin * L;
@@ -132,6 +132,12 @@ __global__ void __launch_bounds__(MAXE <= 16 ? 512 : 256) k_fft(const KfaFftArgs
         const cx<T> x0 = addx0 ? load_aux<T>(A.out, c) : cx<T>{T(0), T(0)};
         store_elem<T>(A, c, e, lds[(size_t)j * A.inst_stride + lds_index<T>(e)], x0);
     }
+}
+
+template <typename T, int MAXE>
+__global__ void __launch_bounds__(MAXE <= 16 ? 512 : 256) k_fft(const KfaFftArgs A) {
+    extern __shared__ __align__(16) unsigned char kfa_smem[];
+    fft_tile<T, MAXE>(A, (uint64_t)blockIdx.x * A.ipw, reinterpret_cast<cx<T>*>(kfa_smem));
 }
 
 template <typename T, int MAXE>

@@ -15,42 +15,16 @@
 //   B(k) waits until all NA tiles of A(k) have stored their results.
 // A tile signals completion after its stores are acknowledged by L2
 // (s_waitcnt vmcnt(0) + barrier, then a device-scope atomic add); a waiter
-// polls with device-scope atomic RMWs, then invalidates its CU's L1
-// (buffer_inv sc0) before reading the slot.  Each wait targets tiles that
+// polls with device-scope atomic loads, then invalidates its CU's L1 with an
+// agent-scope acquire fence before reading the slot.  Each wait targets tiles that
 // were dequeued earlier by running workgroups, so it always ends; a bounded
 // spin still raises an error flag instead of hanging.
 #pragma once
 #include "kfa_fast_impl.h"
+#include "kfa_sync.h"
 
 namespace kfa {
 
-
-__device__ __forceinline__ uint32_t xcc_id() {
-    return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u; // HW_REG_XCC_ID[3:0]
-}
-
-// thread 0 only: wait until *p >= target (bounded); returns false on timeout
-__device__ __forceinline__ bool wait_geq(uint32_t* p, uint32_t target, uint32_t limit,
-                                         uint32_t* err) {
-    for (uint32_t it = 0;; ++it) {
-        const uint32_t v = __hip_atomic_fetch_add(p, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (v >= target)
-            return true;
-        if (it >= limit) {
-            __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return false;
-        }
-        __builtin_amdgcn_s_sleep(4);
-    }
-}
-
-// all of this workgroup's global stores acknowledged, then one increment
-__device__ __forceinline__ void signal_done(uint32_t* p) {
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (threadIdx.x == 0)
-        __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 template <typename T, int N1, int N2, int IPW>
 __global__ void __launch_bounds__(IPW* P2<N1>::TPI) k_fused4(const FusedArgs F) {
@@ -117,7 +91,7 @@ __global__ void __launch_bounds__(IPW* P2<N1>::TPI) k_fused4(const FusedArgs F) 
             if (threadIdx.x == 0)
                 wait_geq(doneA + t, NA, F.spin_limit, err);
             __syncthreads();
-            asm volatile("buffer_inv sc0" ::: "memory"); // drop this CU's stale L1 lines
+            acquire_agent();
             FastArgs b;
             b.in = ring + (size_t)(tl % F.S) * L;
             b.out = (cx<T>*)F.out + (int64_t)t * F.out_dist;

@@ -3,9 +3,31 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "kfa_common.h"
 
 namespace kfa {
+
+#define KFA_RAG_THREADS 256 // threads per workgroup of the ragged executor
+
+// The launches one plan execution would make, recorded instead of launched
+// (the ragged executor turns them into tasks of one persistent launch).  While
+// a recorder is current on the calling thread, launch_fft, launch_gather_first,
+// launch_rader_dc and launch_real_post append their arguments; the planner
+// avoids every other launcher.
+struct RecOp {
+    uint32_t type, prec, maxe, lds;
+    uint64_t tiles; // workgroup tiles (FFT: fixed later from the executor's shape)
+    uint32_t gx;
+    std::vector<unsigned char> args;
+};
+struct Recorder {
+    std::vector<RecOp> ops;
+    hipError_t add(uint32_t type, int prec, int maxe, uint32_t lds, uint64_t tiles, uint32_t gx,
+                   const void* args, size_t bytes);
+};
+Recorder*& recorder(); // current recorder of this thread (null: launch)
 
 hipError_t launch_fft(int prec, int maxe, const KfaFftArgs& a, uint32_t lds_bytes, hipStream_t s);
 hipError_t launch_fft_f64(int maxe, const KfaFftArgs& a, uint32_t lds_bytes, hipStream_t s);
@@ -18,6 +40,13 @@ hipError_t launch_fused_f32(uint32_t N, const FusedArgs& f, uint32_t groups, hip
 hipError_t launch_fused_f64(uint32_t N, const FusedArgs& f, uint32_t groups, hipStream_t s);
 hipError_t launch_xcc_probe(uint32_t* out, uint32_t groups, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+// ragged executor: allow `lds` bytes of dynamic LDS and report resident
+// workgroups per CU; launch the persistent grid; patch caller pointers
+hipError_t ragged_prepare(int prec, uint32_t lds, int* occupancy);
+hipError_t launch_ragged(int prec, const RagArgs& r, uint32_t groups, uint32_t lds, hipStream_t s);
+hipError_t launch_ragged_patch(void* blob, const uint64_t* off, const uint32_t* which,
+                               const int64_t* delta, uint32_t n, const void* in, void* out,
+                               hipStream_t s);
 inline bool fast_len(uint32_t L) { return L >= 16 && L <= 4096 && (L & (L - 1)) == 0; }
 hipError_t launch_stage_twiddles(int prec, void* tw, const KfaFftArgs& a, hipStream_t s);
 hipError_t launch_twiddle_table(void* out_d, uint64_t count, uint64_t step, uint64_t M,
@@ -32,7 +61,8 @@ hipError_t launch_convert_from_f64(int prec, void* dst, const void* src_d, uint6
                                    double scale, hipStream_t s);
 hipError_t launch_real_post(int prec, const void* Z, int64_t z_dist, void* X, int64_t xdh,
                             int64_t xdl, int64_t xs, uint32_t C, uint32_t h, uint64_t batch,
-                            const void* st, int expand, int conj_out, hipStream_t s);
+                            const void* st, int expand, int conj_out, hipStream_t s,
+                            const int64_t* ztbl = nullptr, const int64_t* xtbl = nullptr);
 hipError_t launch_real_pre(int prec, const void* X, int64_t xdh, int64_t xdl, int64_t xs,
                            uint32_t C, void* Z, int64_t z_dist, uint32_t h, uint64_t batch,
                            const void* st, int conj_in, hipStream_t s);
@@ -50,10 +80,11 @@ hipError_t launch_residual(const void* x_f32, const void* z_f64, void* r_f32, ui
 hipError_t launch_accumulate(void* y_f32, const void* d_f32, uint64_t n, hipStream_t s);
 hipError_t launch_gather_first(int prec, int real_in, const void* in, int64_t dist_hi,
                                int64_t dist_lo, uint32_t C, void* out, uint64_t batch,
-                               hipStream_t s);
+                               hipStream_t s, const int64_t* tbl = nullptr);
 
 hipError_t launch_rader_dc(int prec, const void* x0, int x0_real, const void* A, uint32_t L,
                            void* out, int64_t dh, int64_t dl, uint32_t C, int real_out, int conj_,
-                           double scale, uint64_t batch, hipStream_t s);
+                           double scale, uint64_t batch, hipStream_t s,
+                           const int64_t* tbl = nullptr);
 
 } // namespace kfa

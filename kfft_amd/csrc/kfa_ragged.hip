@@ -1,0 +1,145 @@
+// Ragged executor: a batch of transforms of uneven lengths in one persistent
+// launch fed by per-XCD device work queues.
+//
+// The host records, for every length group, the launches its plan would make
+// (general LDS passes, Rader gathers and DC terms, real-input rebuilds) and
+// turns each launch into a record with `tiles` workgroup tasks.  Records of a
+// group form a chain: every task of record r waits until all tiles of the
+// record before it in the chain are done.  A group and all of its records
+// belong to one queue, and a workgroup serves the queue of the XCD it runs on
+// (hardware register XCC_ID), so every intermediate a chain writes is read
+// back on the same XCD, through the same L2.  Tickets of a queue are ordered
+// level by level (all first records, then all second records, ...), so a task
+// only ever waits for tasks dequeued before it by running workgroups; the
+// wait is bounded and raises an error flag instead of hanging.
+#include "kfa_aux_dev.h"
+#include "kfa_fft_impl.h"
+#include "kfa_kernels.h"
+#include "kfa_sync.h"
+
+namespace kfa {
+
+template <typename T>
+__global__ void __launch_bounds__(KFA_RAG_THREADS) k_ragged(const RagArgs R) {
+    extern __shared__ __align__(16) unsigned char kfa_smem[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(kfa_smem);
+    __shared__ uint32_t sh_ticket;
+
+    const uint32_t q = xcc_id() % R.nq;
+    const uint32_t t0 = R.tq_off[q], nt = R.tq_off[q + 1] - t0;
+    uint32_t* tick = R.ctr + 16 * q;
+    uint32_t* done = R.ctr + 16 * R.nq;
+    uint32_t* err = done + R.nrec;
+    const uint32_t tid = threadIdx.x;
+
+    for (;;) {
+        __syncthreads(); // LDS and sh_ticket reuse
+        if (tid == 0)
+            sh_ticket = __hip_atomic_fetch_add(tick, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const uint32_t t = __builtin_amdgcn_readfirstlane(sh_ticket);
+        if (t >= nt)
+            break;
+        const uint32_t ri = __builtin_amdgcn_readfirstlane(R.tq[2 * (size_t)(t0 + t)]);
+        const uint32_t tile = __builtin_amdgcn_readfirstlane(R.tq[2 * (size_t)(t0 + t) + 1]);
+        const RagRecord& rc = R.rec[ri];
+        if (rc.dep >= 0) {
+            if (tid == 0)
+                wait_geq(done + rc.dep, rc.dep_tiles, R.spin_limit, err);
+            __syncthreads();
+            acquire_agent();
+        }
+        const unsigned char* a = R.blob + rc.args;
+        switch (rc.type) {
+        case KFA_REC_FFT: {
+            const KfaFftArgs& A = *reinterpret_cast<const KfaFftArgs*>(a);
+            fft_tile<T, 32>(A, (uint64_t)tile * A.ipw, lds);
+            break;
+        }
+        case KFA_REC_GATHER: {
+            const GatherArgs& g = *reinterpret_cast<const GatherArgs*>(a);
+            const uint64_t b = (uint64_t)tile * KFA_RAG_THREADS + tid;
+            if (b < g.batch)
+                gather_first_one<T>(g, b);
+            break;
+        }
+        case KFA_REC_DC: {
+            const DcArgs& d = *reinterpret_cast<const DcArgs*>(a);
+            const uint64_t b = (uint64_t)tile * KFA_RAG_THREADS + tid;
+            if (b < d.batch)
+                rader_dc_one<T>(d, b);
+            break;
+        }
+        case KFA_REC_REAL_POST: {
+            const RealPostArgs& p = *reinterpret_cast<const RealPostArgs*>(a);
+            const uint32_t b = tile / rc.gx, kx = tile - b * rc.gx;
+            const uint32_t kk = kx * KFA_RAG_THREADS + tid;
+            if (kk <= p.h / 2)
+                real_post_one<T>(p, kk, b);
+            break;
+        }
+        default:
+            break;
+        }
+        signal_done(done + ri);
+    }
+}
+
+// Writes the caller's base pointers into the argument blob: entry i sets the
+// 64-bit word at byte offset off[i] to (which[i] ? out : in) + delta[i].
+__global__ void k_ragged_patch(unsigned char* blob, const uint64_t* off, const uint32_t* which,
+                               const int64_t* delta, uint32_t n, uint64_t in, uint64_t out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        *reinterpret_cast<uint64_t*>(blob + off[i]) = (which[i] ? out : in) + (uint64_t)delta[i];
+}
+
+template <typename T>
+static hipError_t ragged_prepare_t(uint32_t lds, int* occupancy) {
+    static uint32_t set_lds = 0; // largest dynamic LDS size allowed so far
+    if (lds > set_lds) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_ragged<T>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) {
+            (void)hipGetLastError(); // do not leave the error for the next launch
+            return e;
+        }
+        set_lds = lds;
+    }
+    int blocks = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_ragged<T>,
+                                                                KFA_RAG_THREADS, lds);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return e;
+    }
+    *occupancy = blocks;
+    return hipSuccess;
+}
+
+hipError_t ragged_prepare(int prec, uint32_t lds, int* occupancy) {
+    if (lds > 160 * 1024 - 64)
+        return hipErrorInvalidValue;
+    return prec == 8 ? ragged_prepare_t<double>(lds, occupancy)
+                     : ragged_prepare_t<float>(lds, occupancy);
+}
+
+hipError_t launch_ragged(int prec, const RagArgs& r, uint32_t groups, uint32_t lds, hipStream_t s) {
+    if (prec == 8)
+        hipLaunchKernelGGL(k_ragged<double>, dim3(groups), dim3(KFA_RAG_THREADS), lds, s, r);
+    else
+        hipLaunchKernelGGL(k_ragged<float>, dim3(groups), dim3(KFA_RAG_THREADS), lds, s, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_ragged_patch(void* blob, const uint64_t* off, const uint32_t* which,
+                               const int64_t* delta, uint32_t n, const void* in, void* out,
+                               hipStream_t s) {
+    if (!n)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_ragged_patch, dim3((n + 255) / 256), dim3(256), 0, s,
+                       (unsigned char*)blob, off, which, delta, n, (uint64_t)in, (uint64_t)out);
+    return hipGetLastError();
+}
+
+} // namespace kfa

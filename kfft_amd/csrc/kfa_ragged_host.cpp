@@ -1,0 +1,412 @@
+// Host side of the ragged executor: length groups, queue assignment, the
+// recorded task chains, the argument blob and the captured launch graph.
+//
+// Build:
+//  1. Items are grouped by (length, kind); each class gets the shared plan of
+//     its length (half length for even real input, plus super twiddles).
+//  2. Classes are cut into groups of similar work and assigned to the XCD
+//     queues, largest first, each to the least loaded queue.
+//  3. Every group's execution is recorded against placeholder base pointers
+//     and per-instance offset tables (the caller's layout), with its own
+//     slice of scratch.  General passes are reshaped for the executor's
+//     workgroup (256 threads, up to 32 values per thread per stage).
+//  4. Records, tickets (queue by queue, level by level) and the argument blob
+//     go to the device once.  A run patches the caller's pointers into the
+//     blob, clears the counters and starts the persistent grid; the three
+//     launches are captured once per pointer pair and replayed as a graph.
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+#include <map>
+
+#include "kfa_engine.h"
+#include "kfa_kernels.h"
+
+namespace kfa {
+
+namespace {
+
+// Placeholder bases the groups are recorded against.  Every argument word
+// holding one (plus an offset) is patched with the caller's base before a run.
+constexpr uint64_t TAG_IN = 0x7ff0000000000000ull;
+constexpr uint64_t TAG_OUT = 0x7ff8000000000000ull;
+constexpr uint64_t TAG_SPAN = 1ull << 50;
+
+struct Group {
+    uint32_t n = 0, kind = 0, queue = 0;
+    std::vector<uint32_t> items;
+    double work = 0;
+    std::shared_ptr<Plan1D> plan;
+    const DevBuf* st = nullptr; // super twiddles (even real length)
+    size_t tbl_in = 0, tbl_out = 0;
+    size_t scr_off = 0;
+    std::vector<uint32_t> recs; // record indices in chain order
+};
+
+struct RecGuard {
+    explicit RecGuard(Recorder* r) { recorder() = r; }
+    ~RecGuard() { recorder() = nullptr; }
+};
+
+// relative cost of one transform: data moved per pass times the passes, plus a
+// fixed cost per transform
+double est_work(uint32_t n, uint32_t kind) {
+    const double ln = std::max(1.0, std::log2((double)n));
+    const double passes = n <= 4096 ? 1.0 : 2.0;
+    return (kind == RAGGED_REAL ? 0.5 : 1.0) * (double)n * passes * (1.0 + ln / 8.0) + 256.0;
+}
+
+// executor shape of a recorded general pass: threads per instance when a
+// thread may hold up to 32 values per stage, instances per tile within the
+// workgroup and the LDS budget
+bool rag_shape(KfaFftArgs& A, int prec, const RaggedOptions& o, uint32_t* lds) {
+    uint32_t tpi = 1;
+    for (uint32_t s = 0; s < A.nstages; ++s) {
+        const uint32_t R = A.st[s].radix;
+        const uint32_t per = std::max<uint32_t>(1, 32 / R);
+        const uint32_t need = (A.L / R + per - 1) / per;
+        tpi = std::max(tpi, need);
+    }
+    if (tpi > KFA_RAG_THREADS)
+        return false;
+    const size_t per_inst = (size_t)A.inst_stride * elem_bytes(prec);
+    uint32_t ipw = KFA_RAG_THREADS / tpi;
+    if (o.ipw_cap)
+        ipw = std::min(ipw, o.ipw_cap);
+    while (ipw > 1 && per_inst * ipw > o.lds_budget)
+        --ipw;
+    ipw = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ipw, A.batch));
+    if (per_inst * ipw > 160 * 1024 - 64)
+        return false;
+    A.tpi = tpi;
+    A.ipw = ipw;
+    *lds = (uint32_t)(per_inst * ipw);
+    return true;
+}
+
+} // namespace
+
+RaggedPlan::~RaggedPlan() {
+    if (exec_)
+        (void)hipGraphExecDestroy(exec_);
+    if (graph_)
+        (void)hipGraphDestroy(graph_);
+    if (cap_)
+        (void)hipStreamDestroy(cap_);
+}
+
+bool RaggedPlan::build(const std::vector<RaggedItem>& items, const RaggedOptions& opt) {
+    opt_ = opt;
+    nq_ = xcd_queues();
+    if (!nq_) {
+        set_error("ragged executor: the workgroup-to-XCD mapping could not be established");
+        return false;
+    }
+    const uint32_t cus = num_cus();
+    const size_t esz = elem_bytes(prec_);
+    if (items.empty()) {
+        set_error("ragged executor: empty batch");
+        return false;
+    }
+    // 1. classes and their plans
+    std::map<std::pair<uint32_t, uint32_t>, std::vector<uint32_t>> cls;
+    for (uint32_t i = 0; i < (uint32_t)items.size(); ++i) {
+        const RaggedItem& it = items[i];
+        if (it.n == 0 || it.kind > RAGGED_REAL || (it.kind == RAGGED_REAL && inverse_)) {
+            set_error("ragged executor: item " + std::to_string(i) +
+                      " has a zero length, an unknown kind or asks for an inverse real transform");
+            return false;
+        }
+        cls[{it.n, it.kind}].push_back(i);
+    }
+    std::vector<Group> groups;
+    double total = 0;
+    for (auto& kv : cls)
+        total += (double)kv.second.size() * est_work(kv.first.first, kv.first.second);
+    const double target = total / (double)(nq_ * std::max<uint32_t>(1, opt.parts_per_queue));
+    for (auto& kv : cls) {
+        const uint32_t n = kv.first.first, kind = kv.first.second;
+        const bool half = kind == RAGGED_REAL && n % 2 == 0;
+        const uint32_t plen = half ? n / 2 : n;
+        std::shared_ptr<Plan1D> plan = get_plan(plen, prec_);
+        if (!plan)
+            return false;
+        plans_.push_back(plan);
+        const DevBuf* st = nullptr;
+        if (half) {
+            tables_.push_back(std::make_unique<DevBuf>());
+            if (!make_real_twiddles(*tables_.back(), plen, prec_, false))
+                return false;
+            st = tables_.back().get();
+        }
+        const std::vector<uint32_t>& v = kv.second;
+        const double w = est_work(n, kind);
+        const uint32_t min_inst = std::max<uint32_t>(1, 4096 / std::max<uint32_t>(1, plen));
+        uint32_t parts = (uint32_t)std::ceil((double)v.size() * w / std::max(target, 1.0));
+        parts = std::max<uint32_t>(1, std::min<uint32_t>(
+                                          parts, (uint32_t)((v.size() + min_inst - 1) / min_inst)));
+        for (uint32_t p = 0; p < parts; ++p) {
+            const size_t a = v.size() * p / parts, b = v.size() * (p + 1) / parts;
+            if (a == b)
+                continue;
+            Group g;
+            g.n = n;
+            g.kind = kind;
+            g.items.assign(v.begin() + a, v.begin() + b);
+            g.work = w * (double)(b - a);
+            g.plan = plan;
+            g.st = st;
+            groups.push_back(std::move(g));
+        }
+    }
+    // 2. queues: largest group first onto the least loaded queue
+    std::sort(groups.begin(), groups.end(),
+              [](const Group& x, const Group& y) { return x.work > y.work; });
+    std::vector<double> load(nq_, 0.0);
+    for (Group& g : groups) {
+        const uint32_t q = (uint32_t)(std::min_element(load.begin(), load.end()) - load.begin());
+        g.queue = q;
+        load[q] += g.work;
+    }
+    // 3. offset tables (elements of each side's type) and scratch slices
+    std::vector<int64_t> tbl;
+    size_t scr_total = 0;
+    for (Group& g : groups) {
+        const bool real_odd = g.kind == RAGGED_REAL && g.n % 2 == 1;
+        const size_t in_unit = real_odd ? (size_t)prec_ : esz;
+        g.tbl_in = tbl.size();
+        for (uint32_t i : g.items) {
+            if (items[i].in_off % in_unit) {
+                set_error("ragged executor: input offset of item " + std::to_string(i) +
+                          " is not a multiple of " + std::to_string(in_unit) + " bytes");
+                return false;
+            }
+            tbl.push_back((int64_t)(items[i].in_off / in_unit));
+        }
+        g.tbl_out = tbl.size();
+        for (uint32_t i : g.items) {
+            if (items[i].out_off % esz) {
+                set_error("ragged executor: output offset of item " + std::to_string(i) +
+                          " is not a multiple of " + std::to_string(esz) + " bytes");
+                return false;
+            }
+            tbl.push_back((int64_t)(items[i].out_off / esz));
+        }
+        g.scr_off = scr_total;
+        scr_total += (g.plan->scratch_bytes(g.items.size()) + 255) / 256 * 256;
+    }
+    hipStream_t s = default_stream();
+    if (!tbl_.alloc(tbl.size() * 8) || !scratch_.alloc(std::max<size_t>(256, scr_total)) ||
+        !hip_ok(hipMemcpy(tbl_.p, tbl.data(), tbl.size() * 8, hipMemcpyHostToDevice), "ragged tables"))
+        return false;
+    // 4. record every group's chain
+    std::vector<RagRecord> recs;
+    std::vector<unsigned char> blob;
+    std::vector<uint64_t> p_off;
+    std::vector<uint32_t> p_which;
+    std::vector<int64_t> p_delta;
+    auto patch_word = [&](size_t off) {
+        uint64_t v;
+        memcpy(&v, blob.data() + off, 8);
+        for (uint32_t which = 0; which < 2; ++which) {
+            const uint64_t tag = which ? TAG_OUT : TAG_IN;
+            if (v >= tag && v - tag < TAG_SPAN) {
+                p_off.push_back(off);
+                p_which.push_back(which);
+                p_delta.push_back((int64_t)(v - tag));
+            }
+        }
+    };
+    lds_ = 64;
+    size_t nrad16 = 0;
+    for (Group& g : groups) {
+        const uint64_t cnt = g.items.size();
+        const int64_t* tp = (const int64_t*)tbl_.p;
+        Layout in, out;
+        in.ptr = (void*)TAG_IN;
+        in.inst = tp + g.tbl_in;
+        out.ptr = (void*)TAG_OUT;
+        out.inst = tp + g.tbl_out;
+        void* scr = (char*)scratch_.p + g.scr_off;
+        Recorder rec;
+        hipError_t e;
+        {
+            RecGuard guard(&rec);
+            if (g.kind == RAGGED_COMPLEX) {
+                const double scale = (inverse_ && norm_) ? 1.0 / (double)g.n : 1.0;
+                e = g.plan->execute(in, out, cnt, inverse_, scale, scr, s);
+            } else if (g.n % 2) {
+                in.flags = KFA_IO_REAL;
+                e = g.plan->execute(in, out, cnt, false, 1.0, scr, s);
+            } else {
+                e = g.plan->execute(in, out, cnt, false, 1.0, scr, s);
+                if (e == hipSuccess)
+                    e = launch_real_post(prec_, (void*)TAG_OUT, 0, (void*)TAG_OUT, 0, 0, 1, 1,
+                                         g.n / 2, cnt, g.st->p, 0, 0, s, out.inst, out.inst);
+            }
+        }
+        if (e != hipSuccess) {
+            set_error(std::string("ragged executor: recording length ") + std::to_string(g.n) +
+                      " failed (" + hipGetErrorString(e) + ")");
+            return false;
+        }
+        for (size_t k = 0; k < rec.ops.size(); ++k) {
+            RecOp& op = rec.ops[k];
+            RagRecord r;
+            memset(&r, 0, sizeof(r));
+            r.type = op.type;
+            r.gx = op.gx;
+            if (op.type == KFA_REC_FFT) {
+                KfaFftArgs* A = (KfaFftArgs*)op.args.data();
+                uint32_t lds = 0;
+                if (!rag_shape(*A, prec_, opt_, &lds)) {
+                    set_error("ragged executor: a pass of length " + std::to_string(A->L) +
+                              " does not fit the executor's workgroup");
+                    return false;
+                }
+                lds_ = std::max(lds_, lds);
+                r.tiles = (uint32_t)((A->batch + A->ipw - 1) / A->ipw);
+                for (uint32_t st = 0; st < A->nstages; ++st)
+                    nrad16 += A->st[st].radix == 16;
+            } else {
+                r.tiles = (uint32_t)op.tiles;
+            }
+            r.dep = k ? (int32_t)g.recs.back() : -1;
+            r.dep_tiles = k ? recs[g.recs.back()].tiles : 0;
+            r.args = blob.size();
+            blob.insert(blob.end(), op.args.begin(), op.args.end());
+            blob.resize((blob.size() + 63) / 64 * 64, 0);
+            // caller pointers inside the arguments
+            const size_t b0 = r.args;
+            if (op.type == KFA_REC_FFT) {
+                patch_word(b0 + offsetof(KfaFftArgs, in) + offsetof(KfaIo, ptr));
+                patch_word(b0 + offsetof(KfaFftArgs, out) + offsetof(KfaIo, ptr));
+            } else if (op.type == KFA_REC_GATHER) {
+                patch_word(b0 + offsetof(GatherArgs, in));
+            } else if (op.type == KFA_REC_DC) {
+                patch_word(b0 + offsetof(DcArgs, out));
+            } else if (op.type == KFA_REC_REAL_POST) {
+                patch_word(b0 + offsetof(RealPostArgs, Z));
+                patch_word(b0 + offsetof(RealPostArgs, X));
+            }
+            g.recs.push_back((uint32_t)recs.size());
+            recs.push_back(r);
+        }
+    }
+    nrec_ = (uint32_t)recs.size();
+    // 5. tickets: queue by queue, level by level
+    std::vector<uint32_t> tq, tq_off(nq_ + 1, 0);
+    for (uint32_t q = 0; q < nq_; ++q) {
+        size_t levels = 0;
+        for (const Group& g : groups)
+            if (g.queue == q)
+                levels = std::max(levels, g.recs.size());
+        for (size_t lev = 0; lev < levels; ++lev)
+            for (const Group& g : groups) {
+                if (g.queue != q || lev >= g.recs.size())
+                    continue;
+                const uint32_t r = g.recs[lev];
+                for (uint32_t t = 0; t < recs[r].tiles; ++t) {
+                    tq.push_back(r);
+                    tq.push_back(t);
+                }
+            }
+        tq_off[q + 1] = (uint32_t)(tq.size() / 2);
+    }
+    ntick_ = tq.size() / 2;
+    npatch_ = (uint32_t)p_off.size();
+    ctr_words_ = 16 * (size_t)nq_ + nrec_ + 1;
+    auto up = [&](DevBuf& d, const void* h, size_t bytes, const char* what) {
+        return d.alloc(std::max<size_t>(bytes, 8)) &&
+               (bytes == 0 || hip_ok(hipMemcpy(d.p, h, bytes, hipMemcpyHostToDevice), what));
+    };
+    if (!up(rec_, recs.data(), recs.size() * sizeof(RagRecord), "ragged records") ||
+        !up(blob_, blob.data(), blob.size(), "ragged blob") ||
+        !up(tq_, tq.data(), tq.size() * 4, "ragged tickets") ||
+        !up(tq_off_, tq_off.data(), tq_off.size() * 4, "ragged queues") ||
+        !up(patch_off_, p_off.data(), p_off.size() * 8, "ragged patches") ||
+        !up(patch_which_, p_which.data(), p_which.size() * 4, "ragged patches") ||
+        !up(patch_delta_, p_delta.data(), p_delta.size() * 8, "ragged patches") ||
+        !ctr_.alloc(ctr_words_ * 4))
+        return false;
+    // 6. persistent grid
+    int occ = 1;
+    if (!hip_ok(ragged_prepare(prec_, lds_, &occ), "ragged executor LDS"))
+        return false;
+    uint32_t wpc = (uint32_t)std::max(1, occ);
+    if (opt_.wgs_per_cu)
+        wpc = std::min(wpc, opt_.wgs_per_cu);
+    groups_ = std::max(nq_, wpc * std::max<uint32_t>(1, cus));
+    args_.rec = (const RagRecord*)rec_.p;
+    args_.blob = (const unsigned char*)blob_.p;
+    args_.tq = (const uint32_t*)tq_.p;
+    args_.tq_off = (const uint32_t*)tq_off_.p;
+    args_.ctr = (uint32_t*)ctr_.p;
+    args_.nq = nq_;
+    args_.nrec = nrec_;
+    args_.spin_limit = 1u << 22;
+    if (!hip_ok(hipStreamCreateWithFlags(&cap_, hipStreamNonBlocking), "ragged capture stream"))
+        return false;
+    char buf[320];
+    snprintf(buf, sizeof buf,
+             "ragged %zu transforms, %zu classes, %zu groups, %u records, %zu tasks, %u queues, "
+             "%u workgroups x %u threads, %u B LDS, %zu radix-16 stages",
+             items.size(), cls.size(), groups.size(), nrec_, ntick_, nq_, groups_,
+             (uint32_t)KFA_RAG_THREADS, lds_, nrad16);
+    desc_ = buf;
+    return hip_ok(hipStreamSynchronize(s), "ragged build");
+}
+
+hipError_t RaggedPlan::enqueue(const void* in, void* out, hipStream_t s) {
+    hipError_t e = launch_ragged_patch(blob_.p, (const uint64_t*)patch_off_.p,
+                                       (const uint32_t*)patch_which_.p,
+                                       (const int64_t*)patch_delta_.p, npatch_, in, out, s);
+    if (e == hipSuccess)
+        e = launch_fill_u32((uint32_t*)ctr_.p, ctr_words_, 0u, s);
+    if (e == hipSuccess)
+        e = launch_ragged(prec_, args_, groups_, lds_, s);
+    return e;
+}
+
+hipError_t RaggedPlan::run(const void* in, void* out, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(mu_);
+    last_ = s;
+    if (!opt_.graph)
+        return enqueue(in, out, s);
+    if (!exec_ || in != g_in_ || out != g_out_) {
+        if (exec_)
+            (void)hipGraphExecDestroy(exec_);
+        if (graph_)
+            (void)hipGraphDestroy(graph_);
+        exec_ = nullptr;
+        graph_ = nullptr;
+        hipError_t e = hipStreamBeginCapture(cap_, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess)
+            return e;
+        hipError_t e2 = enqueue(in, out, cap_);
+        e = hipStreamEndCapture(cap_, &graph_);
+        if (e2 != hipSuccess)
+            return e2;
+        if (e != hipSuccess)
+            return e;
+        e = hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0);
+        if (e != hipSuccess)
+            return e;
+        g_in_ = in;
+        g_out_ = out;
+    }
+    return hipGraphLaunch(exec_, s);
+}
+
+int RaggedPlan::error() {
+    std::lock_guard<std::mutex> lk(mu_);
+    uint32_t v = 0;
+    if (hipStreamSynchronize(last_) != hipSuccess ||
+        hipMemcpy(&v, (uint32_t*)ctr_.p + 16 * (size_t)nq_ + nrec_, 4, hipMemcpyDeviceToHost) !=
+            hipSuccess)
+        return -1;
+    return (int)v;
+}
+
+} // namespace kfa
