@@ -112,6 +112,106 @@ def _measured_traffic(form: str, n: int, batch: int):
         return None
 
 
+def ragged_bench(count: int, steps: int, warmup: int, per_item: bool = True):
+    """The batching form on the ragged workload: `count` transforms, lengths
+    log-uniform over [16, 65536] (15% forced prime, 30% powers of two), half
+    complex and half real input, FP32, one persistent launch per batch.  Also
+    times the same transforms launched one by one through the batched API."""
+    import torch
+
+    import kfft_amd as K
+    from kfft_amd.workloads import ragged_batch
+
+    lengths, kinds = ragged_batch(count)
+    t0 = time.perf_counter()
+    plan = K.RaggedPlan(lengths, kinds, half=True)
+    plan_s = time.perf_counter() - t0
+    g = torch.Generator(device="cuda").manual_seed(20260105)
+    x = torch.rand((plan.in_bytes + 3) // 4, generator=g, device="cuda") * 2 - 1
+    y = torch.zeros((plan.out_bytes + 3) // 4, device="cuda")
+    torch.cuda.synchronize()
+    stream = torch.cuda.ExternalStream(K.load(True).kfft_gpu_stream())
+    for _ in range(warmup):
+        plan.run_async(x, y)
+    K.synchronize(True)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        plan.run_async(x, y)
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    if plan.error():
+        raise RuntimeError("ragged batch: a dependency wait timed out")
+    out_len = [(n // 2 + 1) if (k == 1 and n % 2 == 0) else n for n, k in zip(lengths, kinds)]
+    alg = float(sum(n * (4 if k == 1 else 8) for n, k in zip(lengths, kinds)) +
+                8 * sum(out_len))
+    flops = float(sum((2.5 if k == 1 else 5.0) * n * math.log2(n) for n, k in zip(lengths, kinds)))
+    res = {"transforms": int(count), "ms_per_batch": round(ms, 4),
+           "gflops": round(flops / (ms * 1e-3) / 1e9, 2),
+           "achieved_gbs": round(alg / (ms * 1e-3) / 1e9, 1),
+           "hbm_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "alg_bytes": alg, "plan_s": round(plan_s, 2), "plan": plan.describe()}
+    # shapes ranked by the committed cost model (trained on this GPU's timings)
+    model_path = os.path.join(ROOT, "kfft_amd", "data", "cost_model_mi355x.npz")
+    if os.path.exists(model_path):
+        import numpy as np
+
+        md = np.load(model_path)
+        m = K.Mlp(int(md["nin"]), int(md["hidden"]))
+        m.set_params(md["params"])
+        m.use_as_cost_model()
+        try:
+            t0 = time.perf_counter()
+            tuned = K.RaggedPlan(lengths, kinds, flags=K.FLAG_GPU_TUNED, half=True)
+            tplan_s = time.perf_counter() - t0
+        finally:
+            K.clear_cost_model()
+        for _ in range(warmup):
+            tuned.run_async(x, y)
+        K.synchronize(True)
+        e0.record(stream)
+        for _ in range(steps):
+            tuned.run_async(x, y)
+        e1.record(stream)
+        e1.synchronize()
+        tms = e0.elapsed_time(e1) / steps
+        if tuned.error():
+            raise RuntimeError("tuned ragged batch: a dependency wait timed out")
+        changed = sum(1 for c in tuned.classes() if tuple(c[3:]) != (16, 0, 0))
+        res["tuned"] = {"ms_per_batch": round(tms, 4), "speedup_vs_default": round(ms / tms, 3),
+                        "plan_s": round(tplan_s, 2), "classes_not_default": changed,
+                        "model": "kfft_amd/data/cost_model_mi355x.npz"}
+    if per_item:
+        # the method without the executor: one asynchronous batched call per transform
+        plans = {}
+        for n, k in set(zip(lengths.tolist(), kinds.tolist())):
+            plans[(n, k)] = K.ScalarPlan(n, half=True) if k == 1 else K.CpxPlan(n, half=True)
+        xs = [x[int(o) // 4:] for o in plan.in_off]
+        ys = [y[int(o) // 4:] for o in plan.out_off]
+        L = K.load(True)
+
+        def one_by_one():
+            for i, (n, k) in enumerate(zip(lengths.tolist(), kinds.tolist())):
+                p = plans[(n, k)]
+                if k == 1:
+                    L.kfft_eval_scalar_batch_async(p.handle, xs[i].data_ptr(), ys[i].data_ptr(),
+                                                   1, None)
+                else:
+                    L.kfft_eval_cpx_batch_async(p.handle, xs[i].data_ptr(), ys[i].data_ptr(), 1,
+                                                None)
+        one_by_one()
+        K.synchronize(True)
+        e0.record(stream)
+        one_by_one()
+        e1.record(stream)
+        e1.synchronize()
+        ms1 = e0.elapsed_time(e1)
+        res["one_launch_per_transform_ms"] = round(ms1, 3)
+        res["speedup_vs_one_launch_per_transform"] = round(ms1 / ms, 2)
+    return res
+
+
 def cpu_baseline(n: int, sample_batch: int):
     import numpy as np
 
@@ -151,6 +251,8 @@ def main():
                     help="comma-separated forms to run (the reported one is always included)")
     ap.add_argument("--cpu-sample", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ragged", type=int, default=20000,
+                    help="transforms of the ragged batching-form workload (0: skip)")
     ap.add_argument("--onchip", action="store_true",
                     help="plan with KFFT_FLAG_GPU_ONCHIP (fused persistent two-pass kernel)")
     ap.add_argument("--copy-ref", action="store_true",
@@ -272,6 +374,9 @@ def main():
         torch.cuda.synchronize()
         out["copy_ref_bytes"] = 2 * x64.numel() * x64.element_size()
         del ref
+    if args.ragged:
+        out["ragged"] = ragged_bench(args.ragged, max(3, args.steps // 2), 1)
+        print(f"# ragged: {out['ragged']['ms_per_batch']} ms per batch", file=sys.stderr, flush=True)
     traffic = None if args.onchip else _measured_traffic(form, n, batch)
     if traffic is not None:
         out["roofline"]["traffic"] = traffic

@@ -75,3 +75,46 @@ KFFT_API kfft_return_t kfft_eval_ragged_async(kfft_plan_ragged* plan, const void
 KFFT_API int kfft_ragged_error(kfft_plan_ragged* plan);
 KFFT_API const char* kfft_ragged_describe(const kfft_plan_ragged* plan);
 KFFT_API void kfft_release_ragged(kfft_plan_ragged* plan);
+/* plan shape chosen for each length class (arrays of kfft_ragged_classes'
+ * count entries, each may be NULL); returns the number of classes */
+KFFT_API uint32_t kfft_ragged_classes(const kfft_plan_ragged* plan, uint32_t* nfft, uint8_t* kind,
+                                      uint32_t* count, uint32_t* max_radix, uint32_t* force_four,
+                                      uint32_t* ipw_cap);
+
+/*
+ * Autotuner and cost model of the batching form.  A choice is (max_radix 16 /
+ * 8 / 4: radix-2 levels fused per LDS stage; force_four: two-pass four-step
+ * where one pass would fit; ipw_cap: largest transforms per workgroup tile, 0
+ * for as many as fit).  Times are microseconds per batch on this GPU.
+ */
+/* the candidate choices (arrays may be NULL); returns their number */
+KFFT_API uint32_t kfft_tune_candidates(uint32_t* max_radix, uint32_t* force_four,
+                                       uint32_t* ipw_cap);
+/* time `count` transforms of one class as a ragged batch under one choice, in
+ * throughput terms (the class replicated to at least 2^22 points, the time
+ * charged per `count` transforms): best of `reps` runs, or a negative value
+ * when the choice does not apply */
+KFFT_API double kfft_tune_measure(uint32_t nfft, uint8_t kind, uint32_t count, uint32_t max_radix,
+                                  uint32_t force_four, uint32_t ipw_cap, uint32_t reps);
+/* cost-model inputs of one class and choice (build precision); returns their number */
+KFFT_API uint32_t kfft_cost_features(uint32_t nfft, uint8_t kind, uint32_t count,
+                                     uint32_t max_radix, uint32_t force_four, uint32_t ipw_cap,
+                                     float* out);
+/* MLP nin -> hidden -> hidden -> 1 (ReLU, FP32) with GPU training and inference
+ * kernels; x, y, t are device pointers ([batch x nin] row-major, [batch]) */
+typedef struct kfft_mlp kfft_mlp;
+KFFT_API kfft_mlp* kfft_mlp_create(uint32_t nin, uint32_t hidden, uint32_t seed);
+KFFT_API uint32_t kfft_mlp_param_count(const kfft_mlp* mlp);
+KFFT_API kfft_return_t kfft_mlp_set_params(kfft_mlp* mlp, const float* host);
+KFFT_API kfft_return_t kfft_mlp_get_params(const kfft_mlp* mlp, float* host);
+KFFT_API kfft_return_t kfft_mlp_forward(kfft_mlp* mlp, const float* x, float* y, uint32_t batch);
+/* gradient of the mean squared error into grad_host (param_count floats) */
+KFFT_API kfft_return_t kfft_mlp_gradient(kfft_mlp* mlp, const float* x, const float* t,
+                                         uint32_t batch, float* grad_host, float* loss);
+/* one Adam step (beta 0.9 / 0.999) on the mean squared error */
+KFFT_API kfft_return_t kfft_mlp_train_step(kfft_mlp* mlp, const float* x, const float* t,
+                                           uint32_t batch, float lr, float* loss);
+KFFT_API void kfft_mlp_release(kfft_mlp* mlp);
+/* the model KFFT_FLAG_GPU_TUNED ragged configs rank choices with (it predicts
+ * ln microseconds); NULL: exhaustive timing of every applicable choice */
+KFFT_API kfft_return_t kfft_ragged_set_cost_model(kfft_mlp* mlp);

@@ -49,6 +49,10 @@ enum kfft_eval_flags {
     /* GPU: run a 256x256 two-pass transform as one persistent launch that keeps the
      * intermediate in each XCD's L2 (experimental; library stream only) */
     KFFT_FLAG_GPU_ONCHIP = 1 << 11,
+    /* GPU, ragged batches: pick each length class's plan shape with the cost
+     * model set by kfft_ragged_set_cost_model, or by exhaustive timing when none
+     * is set */
+    KFFT_FLAG_GPU_TUNED = 1 << 12,
 };
 
 /* incs/kfft_types.h:65 */

@@ -30,6 +30,7 @@ FLAG_GPU_SPLIT_F16 = 1 << 8
 FLAG_GPU_SPLIT_BF16 = 1 << 9
 FLAG_GPU_REFINE = 1 << 10
 FLAG_GPU_ONCHIP = 1 << 11
+FLAG_GPU_TUNED = 1 << 12
 
 RET_SUCCESS, RET_ALLOC_FAIL, RET_BUFFER_FAIL, RET_FREE_NULL = 0, 1, 2, 3
 RET_IMPROPER_PLAN, RET_BADARGUMENTS, RET_GPU_FAIL = 4, 5, 6
@@ -59,7 +60,11 @@ EXPORTS = [
     "kfft_eval_scalar_batch_async", "kfft_evali_scalar_batch_async",
     "kfft_eval2_cpx_async",
     "kfft_config_ragged", "kfft_ragged_layout", "kfft_eval_ragged", "kfft_eval_ragged_async",
-    "kfft_ragged_error", "kfft_ragged_describe", "kfft_release_ragged",
+    "kfft_ragged_error", "kfft_ragged_describe", "kfft_release_ragged", "kfft_ragged_classes",
+    "kfft_tune_candidates", "kfft_tune_measure", "kfft_cost_features",
+    "kfft_mlp_create", "kfft_mlp_param_count", "kfft_mlp_set_params", "kfft_mlp_get_params",
+    "kfft_mlp_forward", "kfft_mlp_gradient", "kfft_mlp_train_step", "kfft_mlp_release",
+    "kfft_ragged_set_cost_model",
 ]
 RAGGED_COMPLEX, RAGGED_REAL = 0, 1
 
@@ -147,6 +152,19 @@ def load(half: bool = False):
         "kfft_ragged_error": (C.c_int, [vp]),
         "kfft_ragged_describe": (C.c_char_p, [vp]),
         "kfft_release_ragged": (None, [vp]),
+        "kfft_ragged_classes": (u32, [vp, vp, vp, vp, vp, vp, vp]),
+        "kfft_tune_candidates": (u32, [vp, vp, vp]),
+        "kfft_tune_measure": (C.c_double, [u32, C.c_uint8, u32, u32, u32, u32, u32]),
+        "kfft_cost_features": (u32, [u32, C.c_uint8, u32, u32, u32, u32, vp]),
+        "kfft_mlp_create": (vp, [u32, u32, u32]),
+        "kfft_mlp_param_count": (u32, [vp]),
+        "kfft_mlp_set_params": (ret, [vp, vp]),
+        "kfft_mlp_get_params": (ret, [vp, vp]),
+        "kfft_mlp_forward": (ret, [vp, vp, vp, u32]),
+        "kfft_mlp_gradient": (ret, [vp, vp, vp, u32, vp, vp]),
+        "kfft_mlp_train_step": (ret, [vp, vp, vp, u32, C.c_float, vp]),
+        "kfft_mlp_release": (None, [vp]),
+        "kfft_ragged_set_cost_model": (ret, [vp]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -431,6 +449,98 @@ class RaggedPlan:
 
     def error(self) -> int:
         return self.L.kfft_ragged_error(self._p)
+
+    def classes(self):
+        """Per length class: (n, kind, count, max_radix, force_four, ipw_cap)."""
+        k = self.L.kfft_ragged_classes(self._p, None, None, None, None, None, None)
+        cols = [np.zeros(k, np.uint32), np.zeros(k, np.uint8)] + [np.zeros(k, np.uint32)
+                                                                  for _ in range(4)]
+        self.L.kfft_ragged_classes(self._p, *[_ptr(c) for c in cols])
+        return list(zip(*[c.tolist() for c in cols]))
+
+
+def tune_candidates(half: bool = True):
+    """The autotuner's choices: list of (max_radix, force_four, ipw_cap)."""
+    L = load(half)
+    k = L.kfft_tune_candidates(None, None, None)
+    cols = [np.zeros(k, np.uint32) for _ in range(3)]
+    L.kfft_tune_candidates(*[_ptr(c) for c in cols])
+    return list(zip(*[c.tolist() for c in cols]))
+
+
+def tune_measure(n, kind, count, choice, reps=3, half: bool = True) -> float:
+    """Microseconds per batch of one class under one choice (negative: not applicable)."""
+    return load(half).kfft_tune_measure(n, kind, count, *choice, reps)
+
+
+def cost_features(n, kind, count, choice, half: bool = True) -> np.ndarray:
+    L = load(half)
+    out = np.zeros(L.kfft_cost_features(n, kind, count, *choice, None), np.float32)
+    L.kfft_cost_features(n, kind, count, *choice, _ptr(out))
+    return out
+
+
+class Mlp:
+    """The cost model's MLP (nin -> hidden -> hidden -> 1, ReLU, FP32) on the GPU
+    kernels of the library.  Inputs and targets are torch CUDA float32 tensors."""
+
+    def __init__(self, nin: int, hidden: int = 64, seed: int = 0, half: bool = True):
+        self.L = load(half)
+        self.half, self.nin, self.hidden = half, nin, hidden
+        self._p = self.L.kfft_mlp_create(nin, hidden, seed)
+        if not self._p:
+            raise KfftError(f"kfft_mlp_create failed: {last_error(half)}")
+        self.nparams = self.L.kfft_mlp_param_count(self._p)
+
+    def __del__(self):
+        try:
+            if self._p:
+                self.L.kfft_mlp_release(self._p)
+                self._p = None
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._p
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.nparams, np.float32)
+        check(self.L.kfft_mlp_get_params(self._p, _ptr(p)), "kfft_mlp_get_params", self.half)
+        return p
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparams
+        check(self.L.kfft_mlp_set_params(self._p, _ptr(p)), "kfft_mlp_set_params", self.half)
+
+    def forward(self, x):
+        import torch
+
+        y = torch.empty(x.shape[0], device=x.device, dtype=torch.float32)
+        check(self.L.kfft_mlp_forward(self._p, _ptr(x), _ptr(y), x.shape[0]), "kfft_mlp_forward",
+              self.half)
+        return y
+
+    def gradient(self, x, t):
+        g = np.zeros(self.nparams, np.float32)
+        loss = C.c_float(0)
+        check(self.L.kfft_mlp_gradient(self._p, _ptr(x), _ptr(t), x.shape[0], _ptr(g),
+                                       C.byref(loss)), "kfft_mlp_gradient", self.half)
+        return g, loss.value
+
+    def train_step(self, x, t, lr: float) -> float:
+        loss = C.c_float(0)
+        check(self.L.kfft_mlp_train_step(self._p, _ptr(x), _ptr(t), x.shape[0], lr,
+                                         C.byref(loss)), "kfft_mlp_train_step", self.half)
+        return loss.value
+
+    def use_as_cost_model(self) -> None:
+        check(self.L.kfft_ragged_set_cost_model(self._p), "kfft_ragged_set_cost_model", self.half)
+
+
+def clear_cost_model(half: bool = True) -> None:
+    check(load(half).kfft_ragged_set_cost_model(None), "kfft_ragged_set_cost_model", half)
 
 
 def synchronize(half: bool = False) -> None:

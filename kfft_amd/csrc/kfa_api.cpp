@@ -17,6 +17,9 @@
 #include "../../include/kfft.h"
 #include "kfa_engine.h"
 #include "kfa_kernels.h"
+#include "kfa_mlp.h"
+
+#include <map>
 
 using namespace kfa;
 
@@ -1443,7 +1446,84 @@ struct kfft_plan_ragged {
     std::unique_ptr<RaggedPlan> plan;
     std::vector<uint64_t> in_off, out_off;
     uint64_t in_bytes = 0, out_bytes = 0;
+    // class -> (count, choice)
+    std::map<std::pair<uint32_t, uint32_t>, std::pair<uint32_t, RaggedChoice>> classes;
 };
+
+struct kfft_mlp {
+    Mlp m;
+    std::mutex mu;
+    kfft_mlp(uint32_t nin, uint32_t hidden) : m(nin, hidden) {}
+};
+
+static std::mutex g_model_mu;
+static kfft_mlp* g_model = nullptr;
+
+// Plan shape of every class of a ragged batch: the cost model's best-ranked
+// applicable choice, or the fastest by exhaustive timing when no model is set.
+static bool choose_classes(kfft_plan_ragged* R) {
+    const std::vector<RaggedChoice> cands = tune_candidates();
+    std::lock_guard<std::mutex> lk(g_model_mu);
+    if (!g_model) {
+        for (auto& kv : R->classes) {
+            const uint32_t n = kv.first.first, kind = kv.first.second, cnt = kv.second.first;
+            double best = -1;
+            for (const RaggedChoice& c : cands) {
+                if (!choice_applies(n, kind, PREC, c))
+                    continue;
+                const double t = tune_measure(n, kind, cnt, PREC, c, 3);
+                if (t > 0 && (best < 0 || t < best)) {
+                    best = t;
+                    kv.second.second = c;
+                }
+            }
+        }
+        return true;
+    }
+    // one inference over every (class, applicable choice)
+    std::vector<float> X;
+    std::vector<std::pair<RaggedChoice*, RaggedChoice>> row;
+    for (auto& kv : R->classes)
+        for (const RaggedChoice& c : cands) {
+            if (!choice_applies(kv.first.first, kv.first.second, PREC, c))
+                continue;
+            float f[COST_FEATURES];
+            cost_features(kv.first.first, kv.first.second, kv.second.first, PREC, c, f);
+            X.insert(X.end(), f, f + COST_FEATURES);
+            row.push_back({&kv.second.second, c});
+        }
+    if (row.empty())
+        return true;
+    std::lock_guard<std::mutex> lm(g_model->mu);
+    DevBuf dx, dy;
+    std::vector<float> y(row.size());
+    hipStream_t s = default_stream();
+    if (!dx.alloc(X.size() * 4) || !dy.alloc(y.size() * 4) ||
+        !hip_ok(hipMemcpy(dx.p, X.data(), X.size() * 4, hipMemcpyHostToDevice), "model input") ||
+        !hip_ok(g_model->m.forward_dev((const float*)dx.p, (float*)dy.p, (uint32_t)row.size(), s),
+                "model forward") ||
+        !hip_ok(hipStreamSynchronize(s), "model sync") ||
+        !hip_ok(hipMemcpy(y.data(), dy.p, y.size() * 4, hipMemcpyDeviceToHost), "model output"))
+        return false;
+    // leave the default shape unless the model expects a clear gain (its
+    // errors are ~10% in time, and classes share the GPU inside a batch)
+    std::map<RaggedChoice*, float> best, dflt;
+    for (size_t i = 0; i < row.size(); ++i) {
+        const RaggedChoice& c = row[i].second;
+        if (c.v.maxr == 16 && !c.v.four && c.ipw_cap == 0)
+            dflt[row[i].first] = y[i];
+    }
+    for (size_t i = 0; i < row.size(); ++i) {
+        auto d = dflt.find(row[i].first);
+        const float bar = d == dflt.end() ? 1e30f : d->second - COST_MARGIN;
+        auto it = best.find(row[i].first);
+        if (y[i] < bar && (it == best.end() || y[i] < it->second)) {
+            best[row[i].first] = y[i];
+            *row[i].first = row[i].second;
+        }
+    }
+    return true;
+}
 
 // executor shape, from the environment where set: KFFT_AMD_RAGGED_IPW (largest
 // instances per tile), KFFT_AMD_RAGGED_WGS (workgroups per CU),
@@ -1490,9 +1570,24 @@ KFFT_API kfft_plan_ragged* kfft_config_ragged(uint32_t count, const uint32_t* nf
         R->out_off.push_back(it.out_off);
     }
     const bool inverse = (flags & KFFT_FLAG_INVERSE) != 0;
+    for (const RaggedItem& it : items)
+        R->classes[{it.n, it.kind}].first++;
+    RaggedOptions opt = ragged_options();
+    if (flags & KFFT_FLAG_GPU_TUNED) {
+        if (inverse) {
+            set_error("ragged batch: tuned plans are forward transforms");
+            return nullptr;
+        }
+        if (!choose_classes(R.get()))
+            return nullptr;
+        kfft_plan_ragged* rp = R.get();
+        opt.choose = [rp](uint32_t n, uint32_t kind, uint32_t) {
+            return rp->classes[{n, kind}].second;
+        };
+    }
     R->plan.reset(new (std::nothrow)
                       RaggedPlan(PREC, inverse, !(flags & KFFT_FLAG_DISABLE_NORM)));
-    if (!R->plan || !R->plan->build(items, ragged_options()))
+    if (!R->plan || !R->plan->build(items, opt))
         return nullptr;
     return R.release();
 }
@@ -1542,5 +1637,148 @@ KFFT_API const char* kfft_ragged_describe(const kfft_plan_ragged* plan) {
 }
 
 KFFT_API void kfft_release_ragged(kfft_plan_ragged* plan) { delete plan; }
+
+KFFT_API uint32_t kfft_ragged_classes(const kfft_plan_ragged* plan, uint32_t* nfft, uint8_t* kind,
+                                      uint32_t* count, uint32_t* max_radix, uint32_t* force_four,
+                                      uint32_t* ipw_cap) {
+    if (!plan)
+        return 0;
+    uint32_t i = 0;
+    for (const auto& kv : plan->classes) {
+        if (nfft)
+            nfft[i] = kv.first.first;
+        if (kind)
+            kind[i] = (uint8_t)kv.first.second;
+        if (count)
+            count[i] = kv.second.first;
+        if (max_radix)
+            max_radix[i] = kv.second.second.v.maxr;
+        if (force_four)
+            force_four[i] = kv.second.second.v.four ? 1u : 0u;
+        if (ipw_cap)
+            ipw_cap[i] = kv.second.second.ipw_cap;
+        ++i;
+    }
+    return i;
+}
+
+// ------------------------------------------------------ autotuner, cost model
+static RaggedChoice make_choice(uint32_t max_radix, uint32_t force_four, uint32_t ipw_cap) {
+    RaggedChoice c;
+    c.v.maxr = max_radix >= 16 ? 16u : (max_radix >= 8 ? 8u : 4u);
+    c.v.four = force_four != 0;
+    c.ipw_cap = ipw_cap;
+    return c;
+}
+
+KFFT_API uint32_t kfft_tune_candidates(uint32_t* max_radix, uint32_t* force_four,
+                                       uint32_t* ipw_cap) {
+    const std::vector<RaggedChoice> c = tune_candidates();
+    for (size_t i = 0; i < c.size(); ++i) {
+        if (max_radix)
+            max_radix[i] = c[i].v.maxr;
+        if (force_four)
+            force_four[i] = c[i].v.four ? 1u : 0u;
+        if (ipw_cap)
+            ipw_cap[i] = c[i].ipw_cap;
+    }
+    return (uint32_t)c.size();
+}
+
+KFFT_API double kfft_tune_measure(uint32_t nfft, uint8_t kind, uint32_t count, uint32_t max_radix,
+                                  uint32_t force_four, uint32_t ipw_cap, uint32_t reps) {
+    if (!runtime_init())
+        return -2.0;
+    const RaggedChoice c = make_choice(max_radix, force_four, ipw_cap);
+    if (!choice_applies(nfft, kind, PREC, c))
+        return -1.0;
+    return tune_measure(nfft, kind, count, PREC, c, reps);
+}
+
+KFFT_API uint32_t kfft_cost_features(uint32_t nfft, uint8_t kind, uint32_t count,
+                                     uint32_t max_radix, uint32_t force_four, uint32_t ipw_cap,
+                                     float* out) {
+    if (out)
+        cost_features(nfft, kind, count, PREC, make_choice(max_radix, force_four, ipw_cap), out);
+    return (uint32_t)COST_FEATURES;
+}
+
+KFFT_API kfft_mlp* kfft_mlp_create(uint32_t nin, uint32_t hidden, uint32_t seed) {
+    if (!nin || !hidden || !runtime_init())
+        return nullptr;
+    std::unique_ptr<kfft_mlp> m(new (std::nothrow) kfft_mlp(nin, hidden));
+    if (!m || !m->m.init(seed))
+        return nullptr;
+    return m.release();
+}
+
+KFFT_API uint32_t kfft_mlp_param_count(const kfft_mlp* mlp) {
+    return mlp ? mlp->m.param_count() : 0;
+}
+
+KFFT_API kfft_return_t kfft_mlp_set_params(kfft_mlp* mlp, const float* host) {
+    if (!mlp || !host)
+        return KFFT_RET_BADARGUMENTS;
+    std::lock_guard<std::mutex> lk(mlp->mu);
+    return mlp->m.set_params(host) ? KFFT_RET_SUCCESS : KFFT_RET_GPU_FAIL;
+}
+
+KFFT_API kfft_return_t kfft_mlp_get_params(const kfft_mlp* mlp, float* host) {
+    if (!mlp || !host)
+        return KFFT_RET_BADARGUMENTS;
+    return mlp->m.get_params(host) ? KFFT_RET_SUCCESS : KFFT_RET_GPU_FAIL;
+}
+
+KFFT_API kfft_return_t kfft_mlp_forward(kfft_mlp* mlp, const float* x, float* y, uint32_t batch) {
+    if (!mlp || !x || !y || !batch)
+        return KFFT_RET_BADARGUMENTS;
+    std::lock_guard<std::mutex> lk(mlp->mu);
+    hipStream_t s = entry_stream();
+    return hip_ok(mlp->m.forward_dev(x, y, batch, s), "mlp forward") &&
+                   hip_ok(hipStreamSynchronize(s), "mlp sync")
+               ? KFFT_RET_SUCCESS
+               : KFFT_RET_GPU_FAIL;
+}
+
+KFFT_API kfft_return_t kfft_mlp_gradient(kfft_mlp* mlp, const float* x, const float* t,
+                                         uint32_t batch, float* grad_host, float* loss) {
+    if (!mlp || !x || !t || !batch)
+        return KFFT_RET_BADARGUMENTS;
+    std::lock_guard<std::mutex> lk(mlp->mu);
+    hipStream_t s = entry_stream();
+    float l = 0.0f;
+    if (!hip_ok(mlp->m.gradient_dev(x, t, batch, &l, s), "mlp gradient") ||
+        !hip_ok(hipStreamSynchronize(s), "mlp sync") ||
+        (grad_host && !mlp->m.get_grads(grad_host)))
+        return KFFT_RET_GPU_FAIL;
+    if (loss)
+        *loss = l;
+    return KFFT_RET_SUCCESS;
+}
+
+KFFT_API kfft_return_t kfft_mlp_train_step(kfft_mlp* mlp, const float* x, const float* t,
+                                           uint32_t batch, float lr, float* loss) {
+    if (!mlp || !x || !t || !batch)
+        return KFFT_RET_BADARGUMENTS;
+    std::lock_guard<std::mutex> lk(mlp->mu);
+    hipStream_t s = entry_stream();
+    return hip_ok(mlp->m.train_step(x, t, batch, lr, loss, s), "mlp train") ? KFFT_RET_SUCCESS
+                                                                           : KFFT_RET_GPU_FAIL;
+}
+
+KFFT_API void kfft_mlp_release(kfft_mlp* mlp) {
+    std::lock_guard<std::mutex> lk(g_model_mu);
+    if (g_model == mlp)
+        g_model = nullptr;
+    delete mlp;
+}
+
+KFFT_API kfft_return_t kfft_ragged_set_cost_model(kfft_mlp* mlp) {
+    std::lock_guard<std::mutex> lk(g_model_mu);
+    if (mlp && mlp->m.nin() != (uint32_t)COST_FEATURES)
+        return KFFT_RET_BADARGUMENTS;
+    g_model = mlp;
+    return KFFT_RET_SUCCESS;
+}
 
 } // extern "C"

@@ -86,7 +86,7 @@ static uint32_t pad_index(uint32_t e, int prec) {
     return e + (e >> (prec == 8 ? KFA_PAD_SHIFT_F64 : KFA_PAD_SHIFT_F32));
 }
 
-std::vector<uint32_t> radix_list(uint32_t L) {
+std::vector<uint32_t> radix_list(uint32_t L, uint32_t maxr) {
     std::vector<uint32_t> r;
     if (L <= 1)
         return r;
@@ -107,6 +107,19 @@ std::vector<uint32_t> radix_list(uint32_t L) {
         }
     if (m != 1)
         return {};
+    if (maxr < 16) {
+        // fewer radix-2 levels fused per stage: 8s (or 4s), then the rest
+        const uint32_t k = maxr >= 8 ? 3 : 2;
+        for (uint32_t i = 0; i < a / k; ++i)
+            r.push_back(1u << k);
+        if (a % k)
+            r.push_back(1u << (a % k));
+        for (uint32_t i = 0; i < b; ++i)
+            r.push_back(3);
+        for (uint32_t p : odd)
+            r.push_back(p);
+        return r;
+    }
     const uint32_t q = a / 4, rr = a % 4;
     if (rr == 1 && q > 0) {
         for (uint32_t i = 0; i + 1 < q; ++i)
@@ -132,14 +145,15 @@ std::vector<uint32_t> radix_list(uint32_t L) {
     return r;
 }
 
-static bool lds_shape(uint32_t L, int prec, int* maxe_out, uint32_t* tpi_out, uint32_t* stride_out) {
+static bool lds_shape(uint32_t L, int prec, int* maxe_out, uint32_t* tpi_out, uint32_t* stride_out,
+                      uint32_t maxr = 16) {
     if (L == 1) {
         *maxe_out = 16;
         *tpi_out = 1;
         *stride_out = 1;
         return true;
     }
-    std::vector<uint32_t> rl = radix_list(L);
+    std::vector<uint32_t> rl = radix_list(L, maxr);
     if (rl.empty() || rl.size() > KFA_MAX_STAGES)
         return false;
     const uint32_t rmax = *std::max_element(rl.begin(), rl.end());
@@ -169,14 +183,14 @@ bool lds_capable(uint32_t L, int prec) {
     return lds_shape(L, prec, &m, &t, &s);
 }
 
-bool LdsPass::plan(uint32_t len, int p) {
+bool LdsPass::plan(uint32_t len, int p, uint32_t max_radix) {
     L = len;
     prec = p;
-    if (!lds_shape(L, prec, &maxe, &tpi, &inst_stride)) {
+    if (!lds_shape(L, prec, &maxe, &tpi, &inst_stride, max_radix)) {
         set_error("length " + std::to_string(L) + " does not fit one LDS pass");
         return false;
     }
-    std::vector<uint32_t> rl = radix_list(L);
+    std::vector<uint32_t> rl = radix_list(L, max_radix);
     nstages = (uint32_t)rl.size();
     uint32_t ns = 1, off = 0;
     for (uint32_t s = 0; s < nstages; ++s) {
@@ -249,19 +263,20 @@ bool subfft_capable(uint32_t L, int prec) {
     return split_four(L, prec, &a, &b);
 }
 
-bool SubFft::plan(uint32_t len, int p, bool force_four) {
+bool SubFft::plan(uint32_t len, int p, bool force_four, uint32_t max_radix) {
     L = len;
     prec = p;
+    maxr = max_radix;
     if (!force_four && lds_capable(L, prec) && (L <= direct_max(prec) || !split_four(L, prec, &N1, &N2))) {
         four = false;
-        return direct.plan(L, prec);
+        return direct.plan(L, prec, maxr);
     }
     if (!split_four(L, prec, &N1, &N2)) {
         set_error("length " + std::to_string(L) + " has no two-pass split");
         return false;
     }
     four = true;
-    if (!col.plan(N1, prec) || !row.plan(N2, prec))
+    if (!col.plan(N1, prec, maxr) || !row.plan(N2, prec, maxr))
         return false;
     // two-level table for W_L^m: m = hi*S + lo, S = 2^shift >= sqrt(L)
     uint32_t sh = 0;
@@ -275,7 +290,7 @@ bool SubFft::plan(uint32_t len, int p, bool force_four) {
     if (!hip_ok(launch_twiddle_table(tw4_lo.p, S, 1, L, s), "tw4 lo") ||
         !hip_ok(launch_twiddle_table(tw4_hi.p, nhi, S, L, s), "tw4 hi"))
         return false;
-    if (fast_len(N1) && fast_len(N2)) {
+    if (fast_len(N1) && fast_len(N2) && maxr >= 16) { // shaped passes use the default radices
         DevBuf full64;
         if (!full64.alloc((size_t)L * 16) || !tw4_full.alloc((size_t)L * elem_bytes(prec)))
             return false;
@@ -352,12 +367,23 @@ static FastArgs fast_args(const void* tw, uint64_t batch, double scale) {
     return f;
 }
 
+// Cooperative load/store order of a general pass: consecutive threads on
+// consecutive instances when instances, not elements, are adjacent in memory
+// (columns of a matrix, four-step passes), else along the elements.
 static bool io_jfast(const KfaIo& io) {
-    if (io.stride == 1 || io.stride == -1)
+    if (io.flags & KFA_IO_MAP)
+        return false; // permuted elements: no order is coalesced
+    const int64_t estep = (int64_t)io.e_mul * io.stride;
+    if (estep == 1 || estep == -1)
         return false;
+    int64_t istep;
     if (io.C > 1)
-        return io.dist_lo == 1 || io.dist_lo == -1;
-    return io.dist_hi == 1 || io.dist_hi == -1;
+        istep = io.dist_lo + (int64_t)io.lo_mul * io.stride;
+    else if (io.flags & KFA_IO_INST)
+        return false; // instance bases come from a table
+    else
+        istep = io.dist_hi;
+    return istep == 1 || istep == -1;
 }
 
 static hipError_t run_pass(const LdsPass& P, const KfaIo& in, const KfaIo& out, double scale,
@@ -536,7 +562,7 @@ hipError_t SubFft::run(const KfaIo& load, const KfaIo& store, double scale, doub
                         fast_io_ok(store, KFA_IO_CONJ2 | KFA_IO_SCALE) && scale_post == 1.0;
     const bool rec = recorder() != nullptr; // recording for the ragged executor: general passes
     if (!four) {
-        if (simple && fast_len(L) && !rec) {
+        if (simple && fast_len(L) && !rec && maxr >= 16) {
             const bool rows = load.C == 1 && store.C == 1 && load.stride == 1 && store.stride == 1;
             const bool cols = load.C == store.C && load.dist_lo == 1 && store.dist_lo == 1 &&
                               load.stride != 1 && store.stride != 1;
@@ -750,7 +776,7 @@ bool Plan1D::build() {
             sub_.four = false;
             return sub_.direct.plan(1, prec_);
         }
-        return sub_.plan(n_, prec_);
+        return sub_.plan(n_, prec_, var_.four, var_.maxr);
     }
     if (is_prime(n_) && subfft_capable(n_ - 1, prec_)) {
         kind_ = Kind::Rader;
@@ -776,7 +802,7 @@ bool Plan1D::build() {
             return false;
         if (!fft_table_f64(L, b64.p, bhat_, prec_, 1.0))
             return false;
-        return sub_.plan(L, prec_);
+        return sub_.plan(L, prec_, var_.four, var_.maxr);
     }
     kind_ = Kind::Bluestein;
     M_ = smooth_at_least(2ull * n_ - 1, prec_);
@@ -797,7 +823,78 @@ bool Plan1D::build() {
         return false;
     if (!hip_ok(hipStreamSynchronize(s), "bluestein sync"))
         return false;
-    return sub_.plan(M_, prec_);
+    return sub_.plan(M_, prec_, var_.four, var_.maxr);
+}
+
+void cost_features(uint32_t n, uint32_t kind, uint32_t count, int prec, const RaggedChoice& c,
+                   float* f) {
+    const bool half = kind == RAGGED_REAL && n % 2 == 0;
+    const uint32_t plen = half ? n / 2 : n;
+    uint32_t L = plen;
+    float rader = 0.0f, blue = 0.0f;
+    if (plen > 1 && !subfft_capable(plen, prec)) {
+        if (is_prime(plen) && subfft_capable(plen - 1, prec)) {
+            rader = 1.0f;
+            L = plen - 1;
+        } else {
+            blue = 1.0f;
+            L = smooth_at_least(2ull * plen - 1, prec);
+        }
+    }
+    int e[6] = {0, 0, 0, 0, 0, 0};
+    uint32_t m = std::max<uint32_t>(L, 1);
+    const uint32_t primes[] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31};
+    const int slot[] = {0, 1, 2, 3, 4, 4, 5, 5, 5, 5, 5};
+    for (int i = 0; i < 11; ++i)
+        while (m % primes[i] == 0) {
+            m /= primes[i];
+            ++e[slot[i]];
+        }
+    const float scale[6] = {17.0f, 11.0f, 7.0f, 6.0f, 5.0f, 4.0f};
+    for (int i = 0; i < 6; ++i)
+        f[i] = (float)e[i] / scale[i];
+    f[6] = rader;
+    f[7] = blue;
+    f[8] = (float)std::log2((double)std::max<uint32_t>(n, 1)) / 16.0f;
+    f[9] = (float)std::log2((double)std::max<uint32_t>(L, 1)) / 18.0f;
+    f[10] = (float)std::log2((double)std::max<uint32_t>(count, 1)) / 14.0f;
+    f[11] = kind == RAGGED_REAL ? 1.0f : 0.0f;
+    f[12] = prec == 8 ? 1.0f : 0.0f;
+    f[13] = c.v.maxr >= 16 ? 1.0f : 0.0f;
+    f[14] = (c.v.maxr >= 8 && c.v.maxr < 16) ? 1.0f : 0.0f;
+    f[15] = c.v.maxr < 8 ? 1.0f : 0.0f;
+    f[16] = c.v.four ? 1.0f : 0.0f;
+    f[17] = c.ipw_cap ? (float)std::log2((double)c.ipw_cap) / 8.0f : 1.0f;
+    f[18] = (!c.v.four && L > 1 && lds_capable(L, prec) &&
+             (L <= direct_max(prec) || !split_four(L, prec, &m, &m)))
+                ? 1.0f
+                : 0.0f;
+}
+
+bool choice_applies(uint32_t n, uint32_t kind, int prec, const RaggedChoice& c) {
+    if (!c.v.four)
+        return true;
+    const uint32_t plen = (kind == RAGGED_REAL && n % 2 == 0) ? n / 2 : n;
+    uint32_t L = plen, a, b;
+    if (plen > 1 && !subfft_capable(plen, prec))
+        L = (is_prime(plen) && subfft_capable(plen - 1, prec))
+                ? plen - 1
+                : smooth_at_least(2ull * plen - 1, prec);
+    return L > 1 && split_four(L, prec, &a, &b);
+}
+
+std::vector<RaggedChoice> tune_candidates() {
+    std::vector<RaggedChoice> v;
+    for (uint32_t maxr : {16u, 8u, 4u})
+        for (bool four : {false, true})
+            for (uint32_t cap : {0u, 16u, 4u, 1u}) {
+                RaggedChoice c;
+                c.v.maxr = maxr;
+                c.v.four = four;
+                c.ipw_cap = cap;
+                v.push_back(c);
+            }
+    return v;
 }
 
 const char* Plan1D::describe() const {
@@ -812,6 +909,8 @@ const char* Plan1D::describe() const {
     else
         snprintf(buf, sizeof buf, "%s n=%u inner=%u one-pass", k, n_, sub_.L);
     desc_ = buf;
+    if (var_.maxr < 16)
+        desc_ += " maxradix=" + std::to_string(var_.maxr);
     return desc_.c_str();
 }
 
@@ -918,17 +1017,17 @@ hipError_t Plan1D::execute(const Layout& in, const Layout& out, uint64_t batch, 
 
 // =============================================================== plan registry
 static std::mutex g_reg_mu;
-static std::map<std::pair<uint32_t, int>, std::weak_ptr<Plan1D>> g_reg;
+static std::map<std::tuple<uint32_t, int, uint32_t>, std::weak_ptr<Plan1D>> g_reg;
 
-std::shared_ptr<Plan1D> get_plan(uint32_t n, int prec) {
+std::shared_ptr<Plan1D> get_plan(uint32_t n, int prec, PlanVariant v) {
     std::lock_guard<std::mutex> lk(g_reg_mu);
-    auto key = std::make_pair(n, prec);
+    auto key = std::make_tuple(n, prec, v.key());
     auto it = g_reg.find(key);
     if (it != g_reg.end()) {
         if (auto sp = it->second.lock())
             return sp;
     }
-    auto p = std::make_shared<Plan1D>(n, prec);
+    auto p = std::make_shared<Plan1D>(n, prec, v);
     if (!p->build())
         return nullptr;
     g_reg[key] = p;

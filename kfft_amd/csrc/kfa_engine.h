@@ -18,9 +18,11 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "kfa_common.h"
@@ -76,13 +78,14 @@ struct LdsPass {
     KfaStage st[KFA_MAX_STAGES] = {};
     DevBuf tw; // stage twiddles
 
-    bool plan(uint32_t L, int prec);
+    bool plan(uint32_t L, int prec, uint32_t max_radix = 16);
     uint32_t pick_ipw(bool jfast) const;
     uint32_t lds_bytes(uint32_t ipw) const;
 };
 
-// Radix list for L (empty when L has a prime factor > 31).
-std::vector<uint32_t> radix_list(uint32_t L);
+// Radix list for L (empty when L has a prime factor > 31).  max_radix below 16
+// fuses fewer radix-2 levels per stage (8s or 4s) and splits 9s into 3s.
+std::vector<uint32_t> radix_list(uint32_t L, uint32_t max_radix = 16);
 bool lds_capable(uint32_t L, int prec);
 
 // A smooth length executed as one LDS pass or a four-step pair of passes.
@@ -90,6 +93,7 @@ struct SubFft {
     uint32_t L = 0;
     int prec = 8;
     bool four = false;
+    uint32_t maxr = 16; // radix cap of the passes (16: the default radices)
     LdsPass direct;
     uint32_t N1 = 0, N2 = 0; // four-step: column length N1, row length N2
     LdsPass col, row;
@@ -97,7 +101,7 @@ struct SubFft {
     uint32_t tw4_shift = 0;
     DevBuf tw4_full; // W_L^m, m < L, compute precision (fast four-step passes)
 
-    bool plan(uint32_t L, int prec, bool force_four = false);
+    bool plan(uint32_t L, int prec, bool force_four = false, uint32_t max_radix = 16);
     // scratch (in complex elements) needed for `batch` instances
     size_t scratch_elems(uint64_t batch) const { return four ? (size_t)batch * L : 0; }
     // Run with fully described first-load and last-store descriptors.  For
@@ -130,9 +134,18 @@ bool subfft_capable(uint32_t L, int prec);
 
 enum class Kind { Identity, Direct, Rader, Bluestein };
 
+// Plan shape choices beyond the default (the autotuner's search space):
+// maxr caps the radix of the LDS stages (stage fusion: 16, 8 or 4), four forces
+// the two-pass four-step form where a one-pass plan would fit.
+struct PlanVariant {
+    uint32_t maxr = 16;
+    bool four = false;
+    uint32_t key() const { return maxr | (four ? 0x100u : 0u); }
+};
+
 class Plan1D {
   public:
-    Plan1D(uint32_t n, int prec) : n_(n), prec_(prec) {}
+    Plan1D(uint32_t n, int prec, PlanVariant v = PlanVariant()) : n_(n), prec_(prec), var_(v) {}
     bool build();
     uint32_t n() const { return n_; }
     int prec() const { return prec_; }
@@ -152,6 +165,7 @@ class Plan1D {
   private:
     uint32_t n_;
     int prec_;
+    PlanVariant var_;
     Kind kind_ = Kind::Identity;
     SubFft sub_;  // Direct: the transform; Rader: length n-1; Bluestein: length M
     uint32_t M_ = 0;
@@ -162,7 +176,7 @@ class Plan1D {
 
 // -------------------------------------------------------------- plan registry
 // Plans are cached by (n, precision) and shared; they are immutable after build.
-std::shared_ptr<Plan1D> get_plan(uint32_t n, int prec);
+std::shared_ptr<Plan1D> get_plan(uint32_t n, int prec, PlanVariant v = PlanVariant());
 
 // Super twiddles of the real-input transform: st[i] = exp(-+ i pi ((i+1)/h + 1/2)).
 bool make_real_twiddles(DevBuf& out, uint32_t h, int prec, bool inverse);
@@ -181,13 +195,49 @@ struct RaggedItem {
     uint64_t in_off = 0, out_off = 0;
 };
 
+// plan shape of one length class: plan variant and largest instances per tile
+// (0: the executor's default)
+struct RaggedChoice {
+    PlanVariant v;
+    uint32_t ipw_cap = 0;
+};
+
 struct RaggedOptions {
     uint32_t lds_budget = 80 * 1024; // per workgroup, for choosing instances per tile
     uint32_t ipw_cap = 0;            // largest instances per tile (0: as many as fit)
     uint32_t wgs_per_cu = 0;         // resident workgroups per CU (0: occupancy limit)
     uint32_t parts_per_queue = 4;    // split of the work of a queue into groups (balance)
+    uint32_t lds_floor = 0;          // least dynamic LDS per workgroup (fixes the occupancy)
     bool graph = true;               // replay the launch sequence as a captured hipGraph
+    // per-class choice (autotuner or cost model); default shapes when empty
+    std::function<RaggedChoice(uint32_t n, uint32_t kind, uint32_t count)> choose;
 };
+
+// Cost-model inputs of one length class under one plan choice (COST_FEATURES
+// values): exponents of 2, 3, 5, 7, {11, 13} and {17..31} in the inner
+// transform length (the length itself, p - 1 for Rader, the smooth M for
+// Bluestein), Rader and Bluestein flags, log2 of the length, the inner length
+// and the batch, real input, FP64, the radix cap (one-hot 16 / 8 / 4), forced
+// four-step, instances per tile, and whether the inner transform runs as one
+// pass.  All scaled to about [0, 1].
+constexpr int COST_FEATURES = 19;
+// ln of the predicted speed-up a choice needs over the default shape before the
+// tuned planner takes it (5%)
+constexpr float COST_MARGIN = 0.0488f;
+void cost_features(uint32_t n, uint32_t kind, uint32_t count, int prec, const RaggedChoice& c,
+                   float* out);
+// the choices the autotuner searches for one class, and whether one applies to
+// a class (forced four-step needs a two-pass split of the inner length)
+std::vector<RaggedChoice> tune_candidates();
+bool choice_applies(uint32_t n, uint32_t kind, int prec, const RaggedChoice& c);
+// Time `count` transforms of one class under one choice as a ragged batch: the
+// class is replicated to at least 2^22 points and run at the occupancy of a
+// mixed batch (68 KiB LDS per workgroup), so that it uses the GPU as it would
+// inside a large batch, and the time is charged per `count` transforms
+// (microseconds, best of `reps`), or a negative value when the choice does not
+// apply to the length.
+double tune_measure(uint32_t n, uint32_t kind, uint32_t count, int prec, const RaggedChoice& c,
+                    uint32_t reps);
 
 class RaggedPlan {
   public:

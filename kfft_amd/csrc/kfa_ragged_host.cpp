@@ -42,6 +42,7 @@ struct Group {
     size_t tbl_in = 0, tbl_out = 0;
     size_t scr_off = 0;
     std::vector<uint32_t> recs; // record indices in chain order
+    uint32_t ipw_cap = 0;
 };
 
 struct RecGuard {
@@ -60,7 +61,7 @@ double est_work(uint32_t n, uint32_t kind) {
 // executor shape of a recorded general pass: threads per instance when a
 // thread may hold up to 32 values per stage, instances per tile within the
 // workgroup and the LDS budget
-bool rag_shape(KfaFftArgs& A, int prec, const RaggedOptions& o, uint32_t* lds) {
+bool rag_shape(KfaFftArgs& A, int prec, const RaggedOptions& o, uint32_t cap, uint32_t* lds) {
     uint32_t tpi = 1;
     for (uint32_t s = 0; s < A.nstages; ++s) {
         const uint32_t R = A.st[s].radix;
@@ -72,8 +73,8 @@ bool rag_shape(KfaFftArgs& A, int prec, const RaggedOptions& o, uint32_t* lds) {
         return false;
     const size_t per_inst = (size_t)A.inst_stride * elem_bytes(prec);
     uint32_t ipw = KFA_RAG_THREADS / tpi;
-    if (o.ipw_cap)
-        ipw = std::min(ipw, o.ipw_cap);
+    if (cap)
+        ipw = std::min(ipw, cap);
     while (ipw > 1 && per_inst * ipw > o.lds_budget)
         --ipw;
     ipw = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ipw, A.batch));
@@ -129,9 +130,17 @@ bool RaggedPlan::build(const std::vector<RaggedItem>& items, const RaggedOptions
         const uint32_t n = kv.first.first, kind = kv.first.second;
         const bool half = kind == RAGGED_REAL && n % 2 == 0;
         const uint32_t plen = half ? n / 2 : n;
-        std::shared_ptr<Plan1D> plan = get_plan(plen, prec_);
+        RaggedChoice ch;
+        if (opt.choose)
+            ch = opt.choose(n, kind, (uint32_t)kv.second.size());
+        std::shared_ptr<Plan1D> plan = get_plan(plen, prec_, ch.v);
+        if (!plan && (ch.v.maxr != 16 || ch.v.four)) { // a variant that does not apply
+            ch.v = PlanVariant();
+            plan = get_plan(plen, prec_);
+        }
         if (!plan)
             return false;
+        const uint32_t cap = ch.ipw_cap ? ch.ipw_cap : opt.ipw_cap;
         plans_.push_back(plan);
         const DevBuf* st = nullptr;
         if (half) {
@@ -157,6 +166,7 @@ bool RaggedPlan::build(const std::vector<RaggedItem>& items, const RaggedOptions
             g.work = w * (double)(b - a);
             g.plan = plan;
             g.st = st;
+            g.ipw_cap = cap;
             groups.push_back(std::move(g));
         }
     }
@@ -260,7 +270,7 @@ bool RaggedPlan::build(const std::vector<RaggedItem>& items, const RaggedOptions
             if (op.type == KFA_REC_FFT) {
                 KfaFftArgs* A = (KfaFftArgs*)op.args.data();
                 uint32_t lds = 0;
-                if (!rag_shape(*A, prec_, opt_, &lds)) {
+                if (!rag_shape(*A, prec_, opt_, g.ipw_cap, &lds)) {
                     set_error("ragged executor: a pass of length " + std::to_string(A->L) +
                               " does not fit the executor's workgroup");
                     return false;
@@ -294,6 +304,7 @@ bool RaggedPlan::build(const std::vector<RaggedItem>& items, const RaggedOptions
             recs.push_back(r);
         }
     }
+    lds_ = std::max(lds_, opt_.lds_floor);
     nrec_ = (uint32_t)recs.size();
     // 5. tickets: queue by queue, level by level
     std::vector<uint32_t> tq, tq_off(nq_ + 1, 0);
@@ -397,6 +408,71 @@ hipError_t RaggedPlan::run(const void* in, void* out, hipStream_t s) {
         g_out_ = out;
     }
     return hipGraphLaunch(exec_, s);
+}
+
+double tune_measure(uint32_t n, uint32_t kind, uint32_t count, int prec, const RaggedChoice& c,
+                    uint32_t reps) {
+    if (!n || !count || kind > RAGGED_REAL)
+        return -1.0;
+    const uint32_t plen = (kind == RAGGED_REAL && n % 2 == 0) ? n / 2 : n;
+    // the variant must apply to this length (else the executor would fall back)
+    std::shared_ptr<Plan1D> keep = get_plan(plen, prec, c.v);
+    if (!keep)
+        return -1.0;
+    const size_t esz = elem_bytes(prec);
+    const uint64_t in_b = (uint64_t)n * (kind == RAGGED_REAL ? (size_t)prec : esz);
+    const uint64_t out_n = (kind == RAGGED_REAL && n % 2 == 0) ? n / 2 + 1 : n;
+    const uint64_t in_stride = (in_b + 255) / 256 * 256, out_stride = (out_n * esz + 255) / 256 * 256;
+    // Throughput, not latency: a class shares the GPU with the rest of a batch,
+    // so it is timed as R copies of itself (at least 2^22 points) and charged 1/R.
+    const uint64_t pts = (uint64_t)n * count;
+    uint32_t R = (uint32_t)std::max<uint64_t>(1, ((1ull << 22) + pts - 1) / pts);
+    R = std::max<uint32_t>(1, std::min<uint32_t>(R, 65536 / count));
+    count *= R;
+    std::vector<RaggedItem> items(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        items[i].n = n;
+        items[i].kind = kind;
+        items[i].in_off = i * in_stride;
+        items[i].out_off = i * out_stride;
+    }
+    RaggedOptions opt;
+    opt.choose = [c](uint32_t, uint32_t, uint32_t) { return c; };
+    // the occupancy of a mixed batch, whose largest pass sets the LDS of every
+    // workgroup (about 68 KiB: two workgroups per CU)
+    opt.lds_floor = 68 * 1024;
+    RaggedPlan rp(prec, false, true);
+    DevBuf in, out;
+    if (!rp.build(items, opt) || !in.alloc(in_stride * count) || !out.alloc(out_stride * count) ||
+        !hip_ok(hipMemset(in.p, 0, in_stride * count), "tune input"))
+        return -2.0;
+    hipStream_t s = default_stream();
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess)
+        return -2.0;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return -2.0;
+    }
+    double best = -2.0;
+    if (rp.run(in.p, out.p, s) == hipSuccess && hipStreamSynchronize(s) == hipSuccess) {
+        for (uint32_t r = 0; r < std::max<uint32_t>(reps, 1); ++r) {
+            float ms = 0.0f;
+            if (hipEventRecord(e0, s) != hipSuccess || rp.run(in.p, out.p, s) != hipSuccess ||
+                hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                best = -2.0;
+                break;
+            }
+            if (best < 0 || ms * 1000.0 < best)
+                best = ms * 1000.0;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rp.error())
+        return -2.0;
+    return best > 0 ? best / R : best;
 }
 
 int RaggedPlan::error() {

@@ -1,0 +1,236 @@
+"""The cost model's GPU kernels against the numpy oracle (oracle/mlp.py), and
+the tuned ragged planner: exhaustive autotuning, the model's ranking, its
+regret against exhaustive search and its gain over the default shapes.
+
+Bars: forward and gradients within 1e-4 relative of the float64 oracle (FP32
+kernels); one Adam step within 1e-5; tuned plans compute the same transforms
+(float bar against the FP64 oracle)."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import kfft_amd as K
+import refs as R
+from oracle import mlp as M
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+DATA = os.path.join(os.path.dirname(HERE), "kfft_amd", "data")
+DB = os.path.join(DATA, "tune_db_mi355x.npz")
+MODEL = os.path.join(DATA, "cost_model_mi355x.npz")
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.complex128 if np.iscomplexobj(a) else np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def setup_net(nin=19, h=64, B=300, seed=0):
+    import torch
+
+    rng = np.random.default_rng(seed)
+    m = K.Mlp(nin, h, seed=seed)
+    p = m.get_params().astype(np.float64)
+    X = rng.uniform(0, 1, (B, nin)).astype(np.float32)
+    t = rng.normal(5, 1, B).astype(np.float32)
+    return m, p, X, t, torch.from_numpy(X).cuda(), torch.from_numpy(t).cuda()
+
+
+def test_forward_matches_oracle():
+    m, p, X, t, xd, td = setup_net()
+    y = m.forward(xd).cpu().numpy()
+    ref, _ = M.forward(p, X, 19, 64)
+    assert rel(y, ref) <= 1e-5
+
+
+@pytest.mark.parametrize("B", [1, 37, 300, 2049])
+def test_gradient_matches_oracle(B):
+    m, p, X, t, xd, td = setup_net(B=B, seed=B)
+    g, loss = m.gradient(xd, td)
+    ref = M.gradient(p, X, t, 19, 64)
+    assert rel(g, ref) <= 1e-4
+    assert abs(loss - M.loss(p, X, t, 19, 64)) <= 1e-5 * max(1, M.loss(p, X, t, 19, 64))
+
+
+def test_adam_steps_match_oracle():
+    m, p, X, t, xd, td = setup_net(B=128, seed=3)
+    mm, vv = np.zeros_like(p), np.zeros_like(p)
+    for step in (1, 2, 3):
+        g = M.gradient(p, X, t, 19, 64)
+        p, mm, vv = M.adam(p, g, mm, vv, step, 1e-3)
+        m.train_step(xd, td, 1e-3)
+    assert rel(m.get_params(), p) <= 1e-5
+
+
+def test_training_fits_a_smooth_target():
+    """The GPU training loop drives the loss of a learnable target down."""
+    import torch
+
+    rng = np.random.default_rng(4)
+    X = rng.uniform(0, 1, (4096, 19)).astype(np.float32)
+    w = rng.normal(0, 1, 19)
+    t = (np.sin(X @ w) + 3 * X[:, 0] * X[:, 1]).astype(np.float32)
+    xd, td = torch.from_numpy(X).cuda(), torch.from_numpy(t).cuda()
+    m = K.Mlp(19, 64, seed=5)
+    first = None
+    for s in range(1500):
+        idx = torch.from_numpy(rng.integers(0, 4096, 256)).cuda()
+        loss = m.train_step(xd[idx].contiguous(), td[idx].contiguous(), 3e-3)
+        first = loss if first is None else first
+    y = m.forward(xd).cpu().numpy()
+    assert np.mean((y - t) ** 2) < 0.05 * first
+
+
+def _ragged_io(p, seed):
+    import torch
+
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    x = torch.rand((p.in_bytes + 3) // 4, generator=g, device="cuda") * 2 - 1
+    y = torch.zeros((p.out_bytes + 3) // 4, device="cuda")
+    return x, y
+
+
+def _check_against_oracle(p, x, y, lengths, kinds, idx=None):
+    """Transforms `idx` (default: all) of a ragged plan against the FP64 oracle."""
+    xh, yh = x.cpu().numpy().astype(np.float64), y.cpu().numpy().astype(np.float64)
+    for i in (range(len(lengths)) if idx is None else idx):
+        n, k = int(lengths[i]), int(kinds[i])
+        o, q = int(p.in_off[i]) // 4, int(p.out_off[i]) // 4
+        if k == 1:
+            ref = R.rfft(xh[o:o + n])
+        else:
+            ref = R.cfft(xh[o:o + 2 * n:2] + 1j * xh[o + 1:o + 2 * n:2])
+        m = len(ref)
+        yi = yh[q:q + 2 * m:2] + 1j * yh[q + 1:q + 2 * m:2]
+        assert rel(yi, ref) <= 2e-6 * max(1, math.log2(n)) / 4, (n, k)
+
+
+def test_exhaustive_tuning_picks_the_fastest_choice():
+    """Without a model, a tuned config times every applicable choice of each
+    class and keeps the fastest; the plan still computes the same transforms."""
+    K.clear_cost_model()
+    lengths, kinds = [256, 256, 256, 1000, 4096], [0, 0, 0, 1, 0]
+    p = K.RaggedPlan(lengths, kinds, flags=K.FLAG_GPU_TUNED, half=True)
+    classes = p.classes()
+    assert len(classes) == 3
+    x, y = _ragged_io(p, 1)
+    p(x, y)
+    _check_against_oracle(p, x, y, lengths, kinds)
+    # the chosen shape of the largest class is among the fastest measured ones
+    n, k, cnt, mr, ff, cap = [c for c in classes if c[0] == 256][0]
+    t_pick = K.tune_measure(n, k, cnt, (mr, ff, cap), 5)
+    t_all = [K.tune_measure(n, k, cnt, ch, 5) for ch in K.tune_candidates()]
+    best = min(t for t in t_all if t > 0)
+    assert t_pick <= 1.3 * best
+
+
+@pytest.mark.skipif(not (os.path.exists(DB) and os.path.exists(MODEL)),
+                    reason="timing database or model not collected")
+def test_model_regret_on_held_out_classes():
+    """The committed model (trained on the training split of the committed
+    MI355X timings) ranks the held-out classes' choices: regret in measured
+    time against exhaustive search, and gain over the default shapes."""
+    import torch
+
+    from kfft_amd import tune as T
+
+    db = dict(np.load(DB))
+    md = np.load(MODEL)
+    m = K.Mlp(int(md["nin"]), int(md["hidden"]))
+    m.set_params(md["params"])
+    X = torch.from_numpy(T.features(db).astype(np.float32)).cuda()
+    pred = m.forward(X).cpu().numpy()
+    test = T.split(db)
+    r = T.regret(db, T.choose(pred, db, test))
+    assert r["classes"] >= 100
+    assert r["median_regret"] <= 0.10, r
+    assert r["mean_regret"] <= 0.25, r
+    assert r["geomean_gain_vs_default"] >= 1.0, r
+
+
+@pytest.mark.skipif(not os.path.exists(MODEL), reason="model not collected")
+def test_model_tuned_ragged_batch():
+    """A tuned ragged batch ranked by the model: correct transforms, one
+    inference instead of per-class timing, and no slower than the defaults."""
+    import torch
+
+    from kfft_amd.workloads import ragged_batch
+
+    md = np.load(MODEL)
+    m = K.Mlp(int(md["nin"]), int(md["hidden"]))
+    m.set_params(md["params"])
+    m.use_as_cost_model()
+    try:
+        lengths, kinds = ragged_batch(400, seed=31, hi=30000)
+        tuned = K.RaggedPlan(lengths, kinds, flags=K.FLAG_GPU_TUNED, half=True)
+        base = K.RaggedPlan(lengths, kinds, half=True)
+    finally:
+        K.clear_cost_model()
+    x, y = _ragged_io(tuned, 2)
+    tuned(x, y)
+    pick = np.random.default_rng(0).choice(len(lengths), 25, replace=False)
+    _check_against_oracle(tuned, x, y, lengths, kinds, idx=pick)
+
+    def timed(p):
+        s = torch.cuda.ExternalStream(K.load(True).kfft_gpu_stream())
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        p.run_async(x, y)
+        K.synchronize(True)
+        e0.record(s)
+        for _ in range(5):
+            p.run_async(x, y)
+        e1.record(s)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / 5
+
+    assert timed(tuned) <= 1.10 * timed(base)
+
+
+@pytest.mark.skipif(not os.path.exists(MODEL), reason="model not collected")
+def test_model_replaces_exhaustive_search():
+    """What the network gains over the method without it: the same batch
+    planned by exhaustive timing of every applicable choice and by one model
+    inference.  The model's plan runs within 10% of the exhaustive plan's time
+    while planning at least 10x faster."""
+    import time
+
+    import torch
+
+    from kfft_amd.workloads import ragged_batch
+
+    lengths, kinds = ragged_batch(40, seed=41, hi=20000)
+    K.clear_cost_model()
+    t0 = time.perf_counter()
+    exh = K.RaggedPlan(lengths, kinds, flags=K.FLAG_GPU_TUNED, half=True)
+    t_exh = time.perf_counter() - t0
+    md = np.load(MODEL)
+    m = K.Mlp(int(md["nin"]), int(md["hidden"]))
+    m.set_params(md["params"])
+    m.use_as_cost_model()
+    try:
+        t0 = time.perf_counter()
+        mod = K.RaggedPlan(lengths, kinds, flags=K.FLAG_GPU_TUNED, half=True)
+        t_mod = time.perf_counter() - t0
+    finally:
+        K.clear_cost_model()
+    assert t_mod * 10 <= t_exh, (t_mod, t_exh)
+    x, y = _ragged_io(mod, 5)
+
+    def timed(p):
+        s = torch.cuda.ExternalStream(K.load(True).kfft_gpu_stream())
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        p.run_async(x, y)
+        K.synchronize(True)
+        best = None
+        for _ in range(5):
+            e0.record(s)
+            p.run_async(x, y)
+            e1.record(s)
+            e1.synchronize()
+            t = e0.elapsed_time(e1)
+            best = t if best is None else min(best, t)
+        return best
+
+    assert timed(mod) <= 1.10 * timed(exh)

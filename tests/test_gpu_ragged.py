@@ -14,8 +14,8 @@ import numpy as np
 import pytest
 
 import kfft_amd as K
-import oracle as O
-from kfft_amd.workloads import _is_prime, ragged_batch
+import refs as R
+from kfft_amd.workloads import ragged_batch
 
 pytestmark = pytest.mark.gpu
 
@@ -61,21 +61,8 @@ def item_out(yh, plan, i, half):
     return v[0::2] + 1j * v[1::2]
 
 
-def oracle_defined(n):
-    """False where the reference factors N = p^2 as one Rader prime (its bisection
-    sqrt truncates below p) and returns a wrong spectrum, which the oracle restates."""
-    stages, _ = O.factor(n)
-    return not (len(stages) == 1 and stages[0][0] == n and not _is_prime(n))
-
-
 def reference(x, n, kind, inverse=False):
-    if kind == K.RAGGED_REAL:
-        if n % 2 == 0 and not oracle_defined(n // 2) or n % 2 and not oracle_defined(n):
-            return np.fft.rfft(x) if n % 2 == 0 else np.fft.fft(x)
-        return O.rfft(x)
-    if not oracle_defined(n):
-        return np.fft.ifft(x) if inverse else np.fft.fft(x)
-    return O.fft(x, O.FLAG_INVERSE if inverse else 0)
+    return R.rfft(x) if kind == K.RAGGED_REAL else R.cfft(x, inverse)
 
 
 # (prime squares such as 121 are left out: the oracle restates the reference's
