@@ -249,6 +249,9 @@ def main():
     ap.add_argument("--form", choices=list(FORMS), default="f16")
     ap.add_argument("--forms", default=",".join(FORMS),
                     help="comma-separated forms to run (the reported one is always included)")
+    ap.add_argument("--only", action="store_true",
+                    help="time the reported form alone (no accuracy sets, no ragged batch, no "
+                         "CPU baseline): for profiler runs")
     ap.add_argument("--cpu-sample", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ragged", type=int, default=20000,
@@ -270,6 +273,9 @@ def main():
 
     n, batch = args.n, args.batch
     forms = [f for f in FORMS if f in args.forms.split(",") or f in (args.form, "fp64")]
+    if args.only:
+        forms = [args.form]
+        args.ragged, args.no_cpu = 0, True
     g = torch.Generator(device="cuda").manual_seed(20260103)
     # set A: uniform[-1,1) re/im, rounded to FP32 once so that every form sees the same values
     x32 = torch.view_as_complex(
@@ -293,31 +299,33 @@ def main():
                 torch.linalg.vector_norm(ref)).item()
 
     # accuracy on set A against the FP64 path
-    y64 = res["fp64"]["y"]
+    y64 = res["fp64"]["y"] if "fp64" in res else None
     for form in forms:
-        if form != "fp64":
+        if form != "fp64" and y64 is not None:
             res[form]["err_uniform"] = rel_err(res[form]["y"], y64)
         del res[form]["y"]
     del y64
     # set B: unit noise plus five 1e-6-amplitude tones per transform
-    t = torch.arange(n, device="cuda", dtype=torch.float64)
-    freqs = torch.randint(0, n, (batch, 5), generator=g, device="cuda")
-    xb = x64.clone()
-    for j in range(5):
-        ph = 2 * math.pi * (freqs[:, j:j + 1].to(torch.float64) * t[None, :] % n) / n
-        xb += 1e-6 * torch.polar(torch.ones_like(ph), ph)
-    xb32 = xb.to(torch.complex64)
-    xb = xb32.to(torch.complex128)
-    del x64, t, freqs
-    torch.cuda.synchronize()
-    yb64 = torch.empty_like(xb)
-    K.CpxPlan(n, 0)(xb, out=yb64)
-    yb = torch.empty_like(xb32)
-    for form in forms:
-        if form != "fp64":
-            K.CpxPlan(n, _form_flags(form), half=True)(xb32, out=yb)
-            res[form]["err_tones"] = rel_err(yb, yb64)
-    del yb, yb64, xb, xb32
+    if not args.only:
+        t = torch.arange(n, device="cuda", dtype=torch.float64)
+        freqs = torch.randint(0, n, (batch, 5), generator=g, device="cuda")
+        xb = x64.clone()
+        for j in range(5):
+            ph = 2 * math.pi * (freqs[:, j:j + 1].to(torch.float64) * t[None, :] % n) / n
+            xb += 1e-6 * torch.polar(torch.ones_like(ph), ph)
+        xb32 = xb.to(torch.complex64)
+        xb = xb32.to(torch.complex128)
+        del t, freqs
+        torch.cuda.synchronize()
+        yb64 = torch.empty_like(xb)
+        K.CpxPlan(n, 0)(xb, out=yb64)
+        yb = torch.empty_like(xb32)
+        for form in forms:
+            if form != "fp64":
+                K.CpxPlan(n, _form_flags(form), half=True)(xb32, out=yb)
+                res[form]["err_tones"] = rel_err(yb, yb64)
+        del yb, yb64, xb, xb32
+    del x64
 
     form = args.form
     plan, ms = res[form]["plan"], res[form]["ms"]
@@ -333,7 +341,7 @@ def main():
              "gflops": round(flops / (r["ms"] * 1e-3) / 1e9, 2),
              "hbm_frac": round(2.0 * n * (8 if f != "fp64" else 16) * batch /
                                (r["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-             "speedup_vs_fp64": round(res["fp64"]["ms"] / r["ms"], 3)}
+             "speedup_vs_fp64": round(res["fp64"]["ms"] / r["ms"], 3) if "fp64" in res else None}
         if "err_uniform" in r:
             e["rel_l2_vs_fp64"] = {"uniform": r["err_uniform"], "tones_1e-6": r["err_tones"]}
         if f[:3] in ("f16", "bf1"):

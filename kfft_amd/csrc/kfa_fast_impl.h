@@ -131,27 +131,31 @@ __device__ __forceinline__ void fast_tile(const FastArgs& A, uint64_t first, cx<
     {
         constexpr int R = S::R(0), NB = E / R, STR = L / R;
         const uint64_t inst = first + li;
-        const bool act = inst < A.batch;
-        const cx<T>* src = (const cx<T>*)A.in;
-        int64_t base = 0;
-        if (act) {
-            const uint64_t hi = inst / A.in_C, lo = inst - hi * A.in_C;
-            base = (int64_t)hi * A.in_hi + (int64_t)lo * A.in_lo;
+        // threads past the batch read the tile's first instance (always valid) and
+        // store nothing, so the loads need no per-element branches
+        const uint64_t ld = inst < A.batch ? inst : first;
+        const uint64_t hi = ld / A.in_C, lo = ld - hi * A.in_C;
+        const cx<T>* p0 = (const cx<T>*)A.in +
+                          ((int64_t)hi * A.in_hi + (int64_t)lo * A.in_lo + (int64_t)ll * A.in_stride);
+        const int64_t st = A.in_stride;
+        if (A.flags & KFA_FAST_NT_IN) {
+#pragma unroll
+            for (int i = 0; i < NB; ++i)
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    v[i * R + r] = load_nt(p0 + (int64_t)(i * TPI + r * STR) * st);
+        } else {
+#pragma unroll
+            for (int i = 0; i < NB; ++i)
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    v[i * R + r] = p0[(int64_t)(i * TPI + r * STR) * st];
         }
+        if (A.flags & KFA_FAST_CONJ_IN) {
 #pragma unroll
-        for (int i = 0; i < NB; ++i)
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int e = (int)ll + i * TPI + r * STR;
-                cx<T> x = {T(0), T(0)};
-                if (act) {
-                    const cx<T>* p = src + base + (int64_t)e * A.in_stride;
-                    x = (A.flags & KFA_FAST_NT_IN) ? load_nt(p) : *p;
-                }
-                if (A.flags & KFA_FAST_CONJ_IN)
-                    x.im = -x.im;
-                v[i * R + r] = x;
-            }
+            for (int k = 0; k < E; ++k)
+                v[k].im = -v[k].im;
+        }
 #pragma unroll
         for (int i = 0; i < NB; ++i)
             dft_any<T, R, DKE>(v + i * R, K);
@@ -227,29 +231,44 @@ __device__ __forceinline__ void fast_tile(const FastArgs& A, uint64_t first, cx<
         const uint64_t inst = first + oi;
         if (inst < A.batch) {
             const uint64_t hi = inst / A.out_C, lo = inst - hi * A.out_C;
-            const int64_t base = (int64_t)hi * A.out_hi + (int64_t)lo * A.out_lo;
-            cx<T>* dst = (cx<T>*)A.out;
-            const T sc = (T)A.scale;
+            cx<T>* q0 = (cx<T>*)A.out + ((int64_t)hi * A.out_hi + (int64_t)lo * A.out_lo +
+                                         (int64_t)ol * A.out_stride);
+            const int64_t st = A.out_stride;
+            if (A.flags & KFA_FAST_TW4) {
+                const cx<T>* w0 = (const cx<T>*)A.tw4 + ((uint32_t)ol * A.out_C + (uint32_t)lo);
 #pragma unroll
-            for (int i = 0; i < NB; ++i)
+                for (int i = 0; i < NB; ++i)
 #pragma unroll
-                for (int r = 0; r < RL; ++r) {
-                    const int e = (int)ol + i * TPI + r * STR;
-                    cx<T> y = v[i * RL + r];
-                    if (A.flags & KFA_FAST_TW4)
-                        y = cmul(y, ((const cx<T>*)A.tw4)[(uint32_t)e * A.out_C + (uint32_t)lo]);
-                    if (A.flags & KFA_FAST_SCALE) {
-                        y.re *= sc;
-                        y.im *= sc;
-                    }
-                    if (A.flags & KFA_FAST_CONJ_OUT)
-                        y.im = -y.im;
-                    cx<T>* p = dst + base + (int64_t)e * A.out_stride;
-                    if (A.flags & KFA_FAST_NT_OUT)
-                        store_nt(p, y);
-                    else
-                        *p = y;
+                    for (int r = 0; r < RL; ++r)
+                        v[i * RL + r] = cmul(v[i * RL + r],
+                                             w0[(uint32_t)(i * TPI + r * STR) * A.out_C]);
+            }
+            if (A.flags & KFA_FAST_SCALE) {
+                const T sc = (T)A.scale;
+#pragma unroll
+                for (int k = 0; k < E; ++k) {
+                    v[k].re *= sc;
+                    v[k].im *= sc;
                 }
+            }
+            if (A.flags & KFA_FAST_CONJ_OUT) {
+#pragma unroll
+                for (int k = 0; k < E; ++k)
+                    v[k].im = -v[k].im;
+            }
+            if (A.flags & KFA_FAST_NT_OUT) {
+#pragma unroll
+                for (int i = 0; i < NB; ++i)
+#pragma unroll
+                    for (int r = 0; r < RL; ++r)
+                        store_nt(q0 + (int64_t)(i * TPI + r * STR) * st, v[i * RL + r]);
+            } else {
+#pragma unroll
+                for (int i = 0; i < NB; ++i)
+#pragma unroll
+                    for (int r = 0; r < RL; ++r)
+                        q0[(int64_t)(i * TPI + r * STR) * st] = v[i * RL + r];
+            }
         }
     }
 }
