@@ -6,8 +6,9 @@
 // computed with v_mfma_f32_32x32x16_{f16,bf16} (two K=16 halves, FP32 accumulate).
 // Operands are split so that the products keep ~FP32 accuracy:
 //   SPLIT_F16:  x = hi + lo (FP16 each), F likewise; A_hi B_hi + A_hi B_lo + A_lo B_hi.
-//               FP16's range is handled per butterfly: x is scaled by a power of two
-//               that puts max|x| in [2^14, 2^15) and the outputs are scaled back.
+//               FP16's range is handled per butterfly: outside [2^-1, 2^14), x is
+//               scaled by a power of two that puts max|x| in [2^14, 2^15) and the
+//               outputs are scaled back.
 //   SPLIT_BF16: x = hi + mid + lo (BF16 each, 24 bits), F likewise; the six products
 //               down to 2^-24 (hh, hm, mh, hl, lh, mm).
 // Lane layout of the 32x32x16 instruction (verified on gfx950): lane l holds
@@ -148,13 +149,18 @@ __device__ __forceinline__ kfa_f16v mfma_dft_group(const MfmaConst<SPLIT>& K, co
 // wave must call it together.
 template <int SPLIT>
 __device__ __forceinline__ void dft16_mfma(cx<float>* v, const MfmaConst<SPLIT>& K) {
+    // FP16 range: a butterfly whose largest component lies outside [2^-1, 2^14)
+    // is scaled by a power of two into [2^14, 2^15) and scaled back after the
+    // products (both exact); the wave skips the scaling when no lane needs it
+    bool scaled = false;
     float sc = 1.0f, inv = 1.0f;
     if constexpr (SPLIT == DFT_SPLIT_F16) {
         float mx = 0.0f;
 #pragma unroll
         for (int i = 0; i < 16; ++i)
             mx = fmaxf(mx, fmaxf(fabsf(v[i].re), fabsf(v[i].im)));
-        if (mx > 0.0f) {
+        scaled = __any(!(mx >= 0.5f && mx < 16384.0f));
+        if (scaled && mx > 0.0f) {
             int e = ilogbf(mx);
             e = e > 120 ? 120 : (e < -110 ? -110 : e);
             sc = ldexpf(1.0f, 14 - e);
@@ -165,10 +171,22 @@ __device__ __forceinline__ void dft16_mfma(cx<float>* v, const MfmaConst<SPLIT>&
     float g0r[8], g0i[8], g1r[8], g1i[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        g0r[i] = v[i].re * sc;
-        g0i[i] = v[i].im * sc;
-        g1r[i] = v[8 + i].re * sc;
-        g1i[i] = v[8 + i].im * sc;
+        g0r[i] = v[i].re;
+        g0i[i] = v[i].im;
+        g1r[i] = v[8 + i].re;
+        g1i[i] = v[8 + i].im;
+    }
+    if (scaled) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            g0r[i] *= sc;
+            g0i[i] *= sc;
+            g1r[i] *= sc;
+            g1i[i] *= sc;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
         swap32(g0r[i], g1r[i]);
         swap32(g0i[i], g1i[i]);
     }
@@ -186,10 +204,17 @@ __device__ __forceinline__ void dft16_mfma(cx<float>* v, const MfmaConst<SPLIT>&
     // 12..15 Im y_{8+4h+j-12}
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        v[j] = {a[j] * inv, a[8 + j] * inv};
-        v[8 + j] = {a[4 + j] * inv, a[12 + j] * inv};
-        v[4 + j] = {b[j] * inv, b[8 + j] * inv};
-        v[12 + j] = {b[4 + j] * inv, b[12 + j] * inv};
+        v[j] = {a[j], a[8 + j]};
+        v[8 + j] = {a[4 + j], a[12 + j]};
+        v[4 + j] = {b[j], b[8 + j]};
+        v[12 + j] = {b[4 + j], b[12 + j]};
+    }
+    if (scaled) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            v[j].re *= inv;
+            v[j].im *= inv;
+        }
     }
 }
 
