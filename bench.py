@@ -212,6 +212,146 @@ def ragged_bench(count: int, steps: int, warmup: int, per_item: bool = True):
     return res
 
 
+def _timed(fn, steps: int, warmup: int = 1) -> float:
+    """ms per call of a synchronous call sequence, wall clock (host pointers
+    are never used: every buffer is device memory)."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps * 1e3
+
+
+def _timed_async(plan, x, y, batch, steps, half):
+    """ms per batched asynchronous call, HIP events on the library stream."""
+    import torch
+
+    import kfft_amd as K
+
+    stream = torch.cuda.ExternalStream(K.load(half).kfft_gpu_stream())
+    plan.run_async(x, y, batch)
+    K.synchronize(half)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        plan.run_async(x, y, batch)
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def other_configs(steps: int):
+    """The reference's other workloads at their full sizes, plain FP64 unless
+    stated: time, algorithmic GB/s against the 8 TB/s roofline, and the
+    relative L2 error of a sample against the oracle (or the stated check)."""
+    import numpy as np
+    import torch
+
+    import kfft_amd as K
+    import oracle as O
+
+    out = {}
+
+    def entry(name, ms, alg_bytes, err, **kw):
+        gbs = alg_bytes / (ms * 1e-3) / 1e9
+        out[name] = dict(ms=round(ms, 4), achieved_gbs=round(gbs, 1),
+                         hbm_frac=round(gbs / HBM_PEAK_GBS, 4), rel_err=err, **kw)
+        print(f"# config {name}: {ms:.4f} ms, {gbs:.0f} GB/s, err {err:.2e}", file=sys.stderr,
+              flush=True)
+
+    def cplx(b, n, dt=torch.complex128, seed=1):
+        g = torch.Generator(device="cuda").manual_seed(seed)
+        rt = torch.float64 if dt == torch.complex128 else torch.float32
+        return torch.view_as_complex(torch.rand(b, n, 2, generator=g, device="cuda", dtype=rt)
+                                     * 2 - 1).contiguous()
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    # 4096 x 256 FP64: forward, then inverse with DISABLE_NORM (checked as inverse / N)
+    n, b = 4096, 256
+    x = cplx(b, n)
+    y, z = torch.empty_like(x), torch.empty_like(x)
+    fw, iv = K.CpxPlan(n), K.CpxPlan(n, K.FLAG_INVERSE | K.FLAG_DISABLE_NORM)
+    tf = _timed_async(fw, x, y, b, steps * 10, False)
+    ti = _timed_async(iv, y, z, b, steps * 10, False)
+    err = max(rel(y[:2].cpu().numpy(), O.fft(x[:2].cpu().numpy())),
+              rel((z / n).cpu().numpy(), x.cpu().numpy()))
+    entry("c2c_4096x256_fwd_inv_f64", tf + ti, 2 * 2.0 * n * 16 * b, err,
+          fwd_ms=round(tf, 4), inv_ms=round(ti, 4))
+    del x, y, z
+    # hard lengths, ~256 MB per set
+    for n, b in [(1000, 16384), (6561, 2560), (1009, 16384), (65537, 256), (1 << 20, 16)]:
+        x = cplx(b, n)
+        y = torch.empty_like(x)
+        p = K.CpxPlan(n)
+        ms = _timed_async(p, x, y, b, steps, False)
+        err = rel(y[0].cpu().numpy(), O.fft(x[0].cpu().numpy()))
+        entry(f"c2c_{n}x{b}_f64", ms, 2.0 * n * 16 * b, err, plan=p.describe())
+        del x, y
+    # real input 2^18 x 512
+    n, b = 1 << 18, 512
+    g = torch.Generator(device="cuda").manual_seed(2)
+    xr = torch.rand(b, n, generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+    sp = K.ScalarPlan(n)
+    yr = torch.empty(b, n // 2 + 1, dtype=torch.complex128, device="cuda")
+    L = K.load(False)
+    ms = _timed(lambda: L.kfft_eval_scalar_batch(sp.handle, xr.data_ptr(), yr.data_ptr(), b),
+                steps)
+    err = rel(yr[0].cpu().numpy(), O.rfft(xr[0].cpu().numpy()))
+    entry("r2c_262144x512_f64", ms, b * (n * 8.0 + (n // 2 + 1) * 16.0), err)
+    del xr, yr
+    # 2D 8192 x 8192 complex FP32, forward and inverse
+    n = 8192
+    a = cplx(n, n, torch.complex64, seed=3)
+    f2, i2 = K.Cpx2Plan(n, n, half=True), K.Cpx2Plan(n, n, K.FLAG_INVERSE, half=True)
+    A = torch.empty_like(a)
+    B = torch.empty_like(a)
+    L32 = K.load(True)
+    tf = _timed(lambda: L32.kfft_eval2_cpx(f2.handle, a.data_ptr(), A.data_ptr()), steps)
+    ti = _timed(lambda: L32.kfft_eval2_cpx(i2.handle, A.data_ptr(), B.data_ptr()), steps)
+    err = rel(B.to(torch.complex128).cpu().numpy(), a.to(torch.complex128).cpu().numpy())
+    entry("c2c_2d_8192x8192_f32", tf + ti, 2 * 2.0 * n * n * 8, err, fwd_ms=round(tf, 4),
+          inv_ms=round(ti, 4), check="inverse(forward(x)) against x")
+    del a, A, B
+    # real 2D convolution: 4096^2 image (uniform[0,1)) with a 63^2 Gaussian plus noise kernel,
+    # both zero-padded to 4160^2; the reference's even-y quirk makes the output 2x the
+    # circular convolution.  Checked against an FP64 direct sum on a 64 x 64 crop.
+    N, M, k = 4160, 4096, 63
+    rng = np.random.default_rng(4)
+    img = np.zeros((N, N))
+    img[:M, :M] = rng.uniform(0, 1, (M, M))
+    yy, xx = np.mgrid[:k, :k] - k // 2
+    ker = np.exp(-(xx ** 2 + yy ** 2) / (2 * 10.0 ** 2)) + 0.01 * rng.uniform(-1, 1, (k, k))
+    kp = np.zeros((N, N))
+    kp[:k, :k] = ker
+    r0, c0 = 1000, 2000
+    crop = np.zeros((64, 64))
+    win = img[r0 - k + 1:r0 + 64, c0 - k + 1:c0 + 64]
+    for dy in range(k):
+        for dx in range(k):
+            crop += ker[dy, dx] * win[k - 1 - dy:k - 1 - dy + 64, k - 1 - dx:k - 1 - dx + 64]
+    for half in (True, False):
+        dt = torch.float32 if half else torch.float64
+        ia = torch.from_numpy(img).to(dt).cuda()
+        ka = torch.from_numpy(kp).to(dt).cuda()
+        o = torch.empty_like(ia)
+        cp = K.Conv2Plan(N, N, real=True, half=half)
+        Lh = K.load(half)
+        ms = _timed(lambda: Lh.kfft_eval2_conv_scalar(cp.handle, ia.data_ptr(), ka.data_ptr(),
+                                                      o.data_ptr()), steps)
+        got = o[r0:r0 + 64, c0:c0 + 64].to(torch.float64).cpu().numpy() / 2.0
+        entry(f"conv2d_real_4160x4160_{'f32' if half else 'f64'}", ms, 3.0 * N * N * (4 if half else 8),
+              rel(got, crop), check="64x64 crop against an FP64 direct sum (output / 2)")
+        del ia, ka, o
+    return out
+
+
 def cpu_baseline(n: int, sample_batch: int):
     import numpy as np
 
@@ -256,6 +396,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ragged", type=int, default=20000,
                     help="transforms of the ragged batching-form workload (0: skip)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the reference's other workloads (4096x256, hard lengths, real "
+                         "2^18, 2D 8192^2, real conv2d 4160^2)")
     ap.add_argument("--onchip", action="store_true",
                     help="plan with KFFT_FLAG_GPU_ONCHIP (fused persistent two-pass kernel)")
     ap.add_argument("--copy-ref", action="store_true",
@@ -305,6 +448,14 @@ def main():
             res[form]["err_uniform"] = rel_err(res[form]["y"], y64)
         del res[form]["y"]
     del y64
+    if args.copy_ref:
+        # one plain device copy of the FP64 input (4 GiB at the default size): the
+        # reference for calibrating profiler byte counters
+        ref = torch.empty_like(x64)
+        ref.copy_(x64)
+        torch.cuda.synchronize()
+        copy_ref_bytes = 2 * x64.numel() * x64.element_size()
+        del ref
     # set B: unit noise plus five 1e-6-amplitude tones per transform
     if not args.only:
         t = torch.arange(n, device="cuda", dtype=torch.float64)
@@ -377,11 +528,9 @@ def main():
         "forms": {f: form_entry(f, r) for f, r in res.items()},
     }
     if args.copy_ref:
-        ref = torch.empty_like(x64)
-        ref.copy_(x64)
-        torch.cuda.synchronize()
-        out["copy_ref_bytes"] = 2 * x64.numel() * x64.element_size()
-        del ref
+        out["copy_ref_bytes"] = copy_ref_bytes
+    if not args.no_configs and not args.only:
+        out["configs"] = other_configs(max(2, args.steps // 4))
     if args.ragged:
         out["ragged"] = ragged_bench(args.ragged, max(3, args.steps // 2), 1)
         print(f"# ragged: {out['ragged']['ms_per_batch']} ms per batch", file=sys.stderr, flush=True)
