@@ -17,6 +17,8 @@
 #pragma once
 #include <stdlib.h>
 
+#include <algorithm>
+
 #include <type_traits>
 
 #include "kfa_device.h"
@@ -90,12 +92,15 @@ __device__ __forceinline__ void store_nt(cx<T>* p, cx<T> x) {
 
 // One radix-R butterfly; radix-16 goes to the matrix cores when DK selects a
 // split-operand form (FP32 data, whole waves only).
+// (returns the split-FP16 codelet's range verdict, else false)
 template <typename T, int R, int DK>
-__device__ __forceinline__ void dft_any(cx<T>* v, const MfmaConst<DK>& K) {
-    if constexpr (R == 16 && DK != DFT_VALU && std::is_same<T, float>::value)
-        dft16_mfma<DK>(v, K);
-    else
+__device__ __forceinline__ bool dft_any(cx<T>* v, const MfmaConst<DK>& K, bool check = true) {
+    if constexpr (R == 16 && DK != DFT_VALU && std::is_same<T, float>::value) {
+        return dft16_mfma<DK>(v, K, check);
+    } else {
         dft<T, R>(v);
+        return false;
+    }
 }
 
 // DFT kind a launch may use: the split forms need FP32 data and workgroups of
@@ -127,6 +132,9 @@ __device__ __forceinline__ void fast_tile(const FastArgs& A, uint64_t first, cx<
     const uint32_t sl = (MODE == FAST_ROW) ? tid % TPI : tid / IPW;
 
     cx<T> v[E];
+    // split FP16: whether stage 0's inputs were narrow enough that stage 1 needs
+    // no range test (decided for the whole workgroup at the first barrier)
+    bool narrow0 = true, skip_check1 = false;
     // ---- stage 0: HBM -> registers, DFT_R0, registers -> LDS (or HBM)
     {
         constexpr int R = S::R(0), NB = E / R, STR = L / R;
@@ -158,7 +166,7 @@ __device__ __forceinline__ void fast_tile(const FastArgs& A, uint64_t first, cx<
         }
 #pragma unroll
         for (int i = 0; i < NB; ++i)
-            dft_any<T, R, DKE>(v + i * R, K);
+            narrow0 = dft_any<T, R, DKE>(v + i * R, K) && narrow0;
         if constexpr (NST > 1) {
             cx<T>* my = lds + (size_t)li * SP;
 #pragma unroll
@@ -175,7 +183,10 @@ __device__ __forceinline__ void fast_tile(const FastArgs& A, uint64_t first, cx<
     // ---- stages 1..NST-1: LDS -> registers, twiddle, DFT, -> LDS or HBM
 #pragma unroll
     for (int s = 1; s < NST; ++s) {
-        __syncthreads();
+        if (DKE == DFT_SPLIT_F16 && s == 1 && S::R(0) == 16)
+            skip_check1 = !__syncthreads_or(narrow0 ? 0 : 1);
+        else
+            __syncthreads();
         const int R = S::R(s), NS = S::NS(s), NB = E / R, STR = L / R;
         cx<T>* my = lds + (size_t)si * SP;
 #pragma unroll
@@ -197,7 +208,7 @@ __device__ __forceinline__ void fast_tile(const FastArgs& A, uint64_t first, cx<
         if (R == 16) {
 #pragma unroll
             for (int i = 0; i < E / 16; ++i)
-                dft_any<T, 16, DKE>(v + i * 16, K);
+                dft_any<T, 16, DKE>(v + i * 16, K, !(s == 1 && skip_check1));
         } else if (R == 8) {
 #pragma unroll
             for (int i = 0; i < E / 8; ++i)
@@ -311,8 +322,11 @@ static hipError_t launch_fast_t(const FastArgs& a, hipStream_t s) {
     static_assert(IPW * P2<L>::TPI <= 1024, "workgroup size");
     static bool attr = false;
     if (!attr) {
+        // exactly the dynamic LDS it uses: a kernel with static LDS (a workgroup
+        // reduction) cannot be granted the whole 160 KiB as dynamic LDS
         hipError_t e = hipFuncSetAttribute((const void*)k_fast<T, L, MODE, IPW, DKE>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)std::max<uint32_t>(lds, 1u));
         if (e != hipSuccess) {
             (void)hipGetLastError(); // do not leave the error for the next launch
             return e;

@@ -31,6 +31,23 @@ enum { DFT_VALU = 0, DFT_SPLIT_F16 = 1, DFT_SPLIT_BF16 = 2 };
 typedef _Float16 kfa_h8 __attribute__((ext_vector_type(8)));
 typedef __bf16 kfa_b8 __attribute__((ext_vector_type(8)));
 typedef float kfa_f16v __attribute__((ext_vector_type(16)));
+typedef _Float16 kfa_h2 __attribute__((ext_vector_type(2)));
+typedef float kfa_f2 __attribute__((ext_vector_type(2)));
+
+// x = hi + lo for a pair of FP32 values: hi rounded to FP16 (packed convert),
+// x - hi formed exactly from the FP16 register by mixed-precision FMAs, then
+// rounded to FP16 (packed convert)
+__device__ __forceinline__ void split_f16_pair(float x0, float x1, kfa_h2& hi, kfa_h2& lo) {
+    const kfa_f2 v = {x0, x1};
+    hi = __builtin_convertvector(v, kfa_h2);
+    float l0, l1;
+    asm("v_fma_mix_f32 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]" : "=v"(l0) : "v"(hi), "v"(x0));
+    asm("v_fma_mix_f32 %0, %1, -1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+        : "=v"(l1)
+        : "v"(hi), "v"(x1));
+    const kfa_f2 l = {l0, l1};
+    lo = __builtin_convertvector(l, kfa_h2);
+}
 
 template <int SPLIT>
 struct MfmaConst {
@@ -103,11 +120,18 @@ __device__ __forceinline__ kfa_f16v mfma_dft_group(const MfmaConst<SPLIT>& K, co
     if constexpr (SPLIT == DFT_SPLIT_F16) {
         kfa_h8 rh, rl, ih, il;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            rh[i] = (_Float16)xr[i];
-            rl[i] = (_Float16)(xr[i] - (float)rh[i]);
-            ih[i] = (_Float16)xi[i];
-            il[i] = (_Float16)(xi[i] - (float)ih[i]);
+        for (int i = 0; i < 8; i += 2) {
+            kfa_h2 h, l;
+            split_f16_pair(xr[i], xr[i + 1], h, l);
+            rh[i] = h[0];
+            rh[i + 1] = h[1];
+            rl[i] = l[0];
+            rl[i + 1] = l[1];
+            split_f16_pair(xi[i], xi[i + 1], h, l);
+            ih[i] = h[0];
+            ih[i + 1] = h[1];
+            il[i] = l[0];
+            il[i + 1] = l[1];
         }
         acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(K.a0h[1], rh, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(K.a0h[0], rl, acc, 0, 0, 0);
@@ -146,20 +170,28 @@ __device__ __forceinline__ kfa_f16v mfma_dft_group(const MfmaConst<SPLIT>& K, co
 }
 
 // In-place forward DFT16 of this lane's butterfly v[0..15]; every lane of the
-// wave must call it together.
+// wave must call it together.  check = false skips the FP16 range test (the
+// caller knows every component lies within [2^-1, 2^14)).  Returns whether
+// every lane's largest component lay within [2^-1, 2^9] (after one more
+// radix-16 stage and a unit twiddle it stays below 2^14), always false when
+// the test is skipped.
 template <int SPLIT>
-__device__ __forceinline__ void dft16_mfma(cx<float>* v, const MfmaConst<SPLIT>& K) {
+__device__ __forceinline__ bool dft16_mfma(cx<float>* v, const MfmaConst<SPLIT>& K,
+                                           bool check = true) {
     // FP16 range: a butterfly whose largest component lies outside [2^-1, 2^14)
     // is scaled by a power of two into [2^14, 2^15) and scaled back after the
     // products (both exact); the wave skips the scaling when no lane needs it
-    bool scaled = false;
+    bool scaled = false, narrow = false;
     float sc = 1.0f, inv = 1.0f;
     if constexpr (SPLIT == DFT_SPLIT_F16) {
         float mx = 0.0f;
+        if (check) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i)
-            mx = fmaxf(mx, fmaxf(fabsf(v[i].re), fabsf(v[i].im)));
-        scaled = __any(!(mx >= 0.5f && mx < 16384.0f));
+            for (int i = 0; i < 16; ++i)
+                mx = fmaxf(mx, fmaxf(fabsf(v[i].re), fabsf(v[i].im)));
+            scaled = __any(!(mx >= 0.5f && mx < 16384.0f));
+            narrow = !__any(!(mx >= 0.5f && mx <= 512.0f));
+        }
         if (scaled && mx > 0.0f) {
             int e = ilogbf(mx);
             e = e > 120 ? 120 : (e < -110 ? -110 : e);
@@ -216,6 +248,7 @@ __device__ __forceinline__ void dft16_mfma(cx<float>* v, const MfmaConst<SPLIT>&
             v[j].im *= inv;
         }
     }
+    return narrow;
 }
 
 } // namespace kfa
