@@ -607,10 +607,31 @@ static kfft_return_t sclr_inv(kfft_plan_sclr* P, const kfft_cpx* fin, kfft_scala
 // Column transforms of a y-by-x row-major complex matrix (batch `nmat`),
 // in place: strided LDS passes when the column plan allows it, otherwise
 // transpose / rows / transpose through `work`.
-static kfft_return_t cols_cpx_dev(const Plan1D& py, void* data, uint32_t x, uint32_t y,
+// Columns longer than the shaped one-pass lengths run as a column four-step
+// (two passes whose lanes walk the columns) rather than the one-pass kernel,
+// whose workgroups would each walk a single column.
+static const Plan1D& column_plan(const Plan1D& py) {
+    if (py.kind() != Kind::Direct || py.two_pass() || py.n() <= 4096)
+        return py;
+    static std::mutex mu;
+    static std::map<std::pair<uint32_t, int>, std::shared_ptr<Plan1D>> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto key = std::make_pair(py.n(), py.prec());
+    auto it = cache.find(key);
+    if (it == cache.end()) {
+        PlanVariant v;
+        v.four = true;
+        std::shared_ptr<Plan1D> p = get_plan(py.n(), py.prec(), v);
+        it = cache.emplace(key, (p && p->supports_plain_columns()) ? p : nullptr).first;
+    }
+    return it->second ? *it->second : py;
+}
+
+static kfft_return_t cols_cpx_dev(const Plan1D& py_, void* data, uint32_t x, uint32_t y,
                                   uint32_t nmat, bool inverse, double scale, DevBuf& scratch,
                                   DevBuf& work, hipStream_t s) {
-    if (py.supports_columns()) {
+    const Plan1D& py = column_plan(py_);
+    if (py.supports_plain_columns()) {
         if (!scratch.ensure(py.scratch_bytes((uint64_t)x * nmat)))
             return gpu_fail("scratch");
         Layout l;

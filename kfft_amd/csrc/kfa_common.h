@@ -80,6 +80,7 @@ struct KfaFftArgs {
 #define KFA_FAST_TW4 8u
 #define KFA_FAST_NT_IN 16u  // non-temporal (streaming) loads of the input
 #define KFA_FAST_NT_OUT 32u // non-temporal (streaming) stores of the output
+#define KFA_FAST_TW4_HI 256u    // four-step twiddle indexed by (instance / out_C) mod pad
 #define KFA_FAST_MFMA_F16 64u   // radix-16 stages as split FP16 matrix products (FP32 data)
 #define KFA_FAST_MFMA_BF16 128u // radix-16 stages as split BF16 matrix products (FP32 data)
 
@@ -93,7 +94,7 @@ struct FastArgs {
     uint64_t batch;
     uint32_t in_C, out_C;
     uint32_t flags;
-    uint32_t pad;
+    uint32_t pad; // KFA_FAST_TW4_HI: the four-step row length N2 (twiddle table stride)
     double scale;
 };
 

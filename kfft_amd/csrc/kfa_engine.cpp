@@ -452,6 +452,57 @@ hipError_t SubFft::run_four_fast(const KfaIo& load, const KfaIo& store, double s
     return launch_fast(prec, N2, 2, b, s);
 }
 
+// Column four-step on row-major matrices: instance = column x of matrix m,
+// element n at row n (pitch `stride`).  Pass A: for (n2, x), FFT over n1 of
+// rows n1*N2 + n2, store k1 at row k1*N2 + n2 of the scratch (pitch C) times
+// W_L^(n2 k1); pass B: for (k1, x), FFT over n2, store k2 at row k1 + N1*k2.
+// One launch pair per matrix.
+hipError_t SubFft::run_four_cols(const KfaIo& load, const KfaIo& store, double scale,
+                                 uint64_t batch, void* scratch, uint32_t nt, hipStream_t s,
+                                 uint32_t dft) const {
+    const uint32_t X = load.C;
+    const uint64_t nmat = batch / X;
+    const int64_t esz = (int64_t)elem_bytes(prec);
+    for (uint64_t m = 0; m < nmat; ++m) {
+        FastArgs a = fast_args(col.tw.p, (uint64_t)N2 * X, 1.0);
+        a.in = (const void*)(load.ptr + (uint64_t)((int64_t)m * load.dist_hi * esz));
+        a.in_C = X;
+        a.in_hi = load.stride;
+        a.in_lo = 1;
+        a.in_stride = (int64_t)N2 * load.stride;
+        a.out = scratch;
+        a.out_C = X;
+        a.out_hi = X;
+        a.out_lo = 1;
+        a.out_stride = (int64_t)N2 * X;
+        a.tw4 = tw4_full.p;
+        a.pad = N2;
+        a.flags = KFA_FAST_TW4_HI | ((load.flags & KFA_IO_CONJ1) ? KFA_FAST_CONJ_IN : 0) |
+                  ((nt & 1) ? KFA_FAST_NT_IN : 0) | dft;
+        hipError_t e = launch_fast(prec, N1, 1, a, s);
+        if (e != hipSuccess)
+            return e;
+        FastArgs b = fast_args(row.tw.p, (uint64_t)N1 * X, scale);
+        b.in = scratch;
+        b.in_C = X;
+        b.in_hi = (int64_t)N2 * X;
+        b.in_lo = 1;
+        b.in_stride = X;
+        b.out = (void*)(store.ptr + (uint64_t)((int64_t)m * store.dist_hi * esz));
+        b.out_C = X;
+        b.out_hi = store.stride;
+        b.out_lo = 1;
+        b.out_stride = (int64_t)N1 * store.stride;
+        b.flags = ((store.flags & KFA_IO_CONJ2) ? KFA_FAST_CONJ_OUT : 0) |
+                  ((store.flags & KFA_IO_SCALE) ? KFA_FAST_SCALE : 0) |
+                  ((nt & 2) ? KFA_FAST_NT_OUT : 0) | dft;
+        e = launch_fast(prec, N2, 1, b, s);
+        if (e != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
+}
+
 // ============================================================ fused two-pass
 static uint32_t env_u32(const char* name, uint32_t dflt) {
     const char* e = getenv(name);
@@ -586,6 +637,9 @@ hipError_t SubFft::run(const KfaIo& load, const KfaIo& store, double scale, doub
         }
         return run_pass(direct, load, store, scale, scale_post, batch, nullptr, nullptr, 0, 0, s);
     }
+    if (simple && tw4_full.p && !rec && load.C > 1 && store.C == load.C && load.dist_lo == 1 &&
+        store.dist_lo == 1 && load.stride > 0 && store.stride > 0 && batch % load.C == 0)
+        return run_four_cols(load, store, scale, batch, scratch, tuning().nt, s, dft);
     if (load.C != 1 || store.C != 1)
         return hipErrorInvalidValue;
     if (onchip && !rec && !dft && simple && tw4_full.p && N1 == 256 && N2 == 256 && s == default_stream() &&
@@ -915,6 +969,11 @@ const char* Plan1D::describe() const {
 }
 
 bool Plan1D::supports_columns() const { return !sub_.four; }
+
+bool Plan1D::supports_plain_columns() const {
+    // one-pass plans, and Direct four-step plans on the shaped passes (column four-step)
+    return !sub_.four || (kind_ == Kind::Direct && sub_.tw4_full.p);
+}
 
 size_t Plan1D::scratch_bytes(uint64_t batch) const {
     const size_t eb = elem_bytes(prec_);

@@ -112,6 +112,10 @@ struct SubFft {
                    uint32_t dft = 0) const;
     hipError_t run_four_fast(const KfaIo& load, const KfaIo& store, double scale, uint64_t batch,
                              void* scratch, uint32_t nt, hipStream_t s, uint32_t dft) const;
+    // both passes on columns of row-major matrices (C = columns, dist_lo = 1): the
+    // shaped column passes walk the columns with the lanes
+    hipError_t run_four_cols(const KfaIo& load, const KfaIo& store, double scale, uint64_t batch,
+                             void* scratch, uint32_t nt, hipStream_t s, uint32_t dft) const;
     // fused on-chip variant of the two passes (N1 == N2 == 256, library stream only)
     hipError_t run_fused(const KfaIo& load, const KfaIo& store, double scale, uint64_t batch,
                          hipStream_t s) const;
@@ -159,7 +163,11 @@ class Plan1D {
     // by Rader/Bluestein whose inner length is Direct.
     hipError_t execute(const Layout& in, const Layout& out, uint64_t batch, bool inverse,
                        double scale, void* scratch, hipStream_t s, const Fuse* fuse = nullptr) const;
+    // column layouts (C > 1) with any fused element operations
     bool supports_columns() const;
+    // column layouts of plain complex data (no real / mapped / pointwise element operations)
+    bool supports_plain_columns() const;
+    bool two_pass() const { return sub_.four; }
     int async_error() const { return sub_.fused_error(); }
 
   private:

@@ -245,14 +245,18 @@ __device__ __forceinline__ void fast_tile(const FastArgs& A, uint64_t first, cx<
             cx<T>* q0 = (cx<T>*)A.out + ((int64_t)hi * A.out_hi + (int64_t)lo * A.out_lo +
                                          (int64_t)ol * A.out_stride);
             const int64_t st = A.out_stride;
-            if (A.flags & KFA_FAST_TW4) {
-                const cx<T>* w0 = (const cx<T>*)A.tw4 + ((uint32_t)ol * A.out_C + (uint32_t)lo);
+            if (A.flags & (KFA_FAST_TW4 | KFA_FAST_TW4_HI)) {
+                // W_N^(col * e) from the [k1][n2] table: col is the instance's column
+                // (lo) in a 1D four-step, its row block (hi mod N2) in a column four-step
+                const bool by_hi = (A.flags & KFA_FAST_TW4_HI) != 0;
+                const uint32_t ts = by_hi ? A.pad : A.out_C;
+                const uint32_t col = by_hi ? (uint32_t)(hi % A.pad) : (uint32_t)lo;
+                const cx<T>* w0 = (const cx<T>*)A.tw4 + ((uint32_t)ol * ts + col);
 #pragma unroll
                 for (int i = 0; i < NB; ++i)
 #pragma unroll
                     for (int r = 0; r < RL; ++r)
-                        v[i * RL + r] = cmul(v[i * RL + r],
-                                             w0[(uint32_t)(i * TPI + r * STR) * A.out_C]);
+                        v[i * RL + r] = cmul(v[i * RL + r], w0[(uint32_t)(i * TPI + r * STR) * ts]);
             }
             if (A.flags & KFA_FAST_SCALE) {
                 const T sc = (T)A.scale;

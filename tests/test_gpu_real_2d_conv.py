@@ -79,8 +79,10 @@ def test_real_device_batch():
     assert rel(got, ref) <= tol(n)
 
 
+# (8192, 40) and (16384, 3): columns longer than the one-pass shaped lengths run as a
+# column four-step on the shaped passes (lanes over columns, twiddle by row block)
 SHAPES_2D = [(6, 10), (7, 5), (16, 12), (11, 13), (64, 48), (128, 256), (1009, 8), (8, 2018),
-             (96, 4160)]
+             (96, 4160), (8192, 40), (16384, 3)]
 
 
 @pytest.mark.parametrize("y,x", SHAPES_2D)
@@ -165,3 +167,14 @@ def test_hadamard_transpose_device():
     od = torch.empty(20 * 12, dtype=torch.complex128, device="cuda")
     lib.kfft_math_transpose_cpx(C.c_void_p(bd.data_ptr()), C.c_void_p(od.data_ptr()), 20, 12)
     assert np.array_equal(od.cpu().numpy().reshape(20, 12), b.T)
+
+
+@pytest.mark.parametrize("y,x", [(8192, 40), (8192, 512), (16384, 24)])
+def test_2d_complex_float_long_columns(y, x):
+    """The float build's column four-step (8192 would otherwise run as one pass)."""
+    rng = np.random.default_rng(y + x)
+    a = cplx(rng, y, x).astype(np.complex64)
+    A = K.Cpx2Plan(x, y, half=True)(a)
+    assert rel(A, O.fft2(a.astype(np.complex128))) <= 2e-6 * math.log2(x * y) / 4
+    back = K.Cpx2Plan(x, y, K.FLAG_INVERSE, half=True)(A)
+    assert rel(back, a) <= 2e-6 * math.log2(x * y) / 4
