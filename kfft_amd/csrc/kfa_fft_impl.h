@@ -2,11 +2,11 @@
 // complex values a thread holds per stage (MAXE).
 //
 // A workgroup owns `ipw` consecutive instances of a length-L transform.
-//  1. Cooperative load: the workgroup walks its ipw*L elements, consecutive
-//     threads taking consecutive elements of one instance (row passes) or the
-//     same element of consecutive instances (in_jfast, column passes), applies
-//     the pass's fused load operations and writes the values into a padded LDS
-//     region per instance.
+//  1. Load: tpi threads per instance, each walking elements lane, lane + tpi,
+//     ...; the lanes of one instance sit on consecutive threads (row passes) or
+//     the same lane of consecutive instances does (in_jfast, column passes).
+//     The pass's fused load operations apply on the way into a padded LDS
+//     region per instance; plain layouts advance one address per element.
 //  2. Radix stages, LDS to LDS, `tpi` threads per instance (Stockham autosort):
 //       stage s of radix R, ns = product of earlier radices, butterfly b < L/R:
 //         x_r = lds[b + r L/R] * W_{ns R}^{r (b mod ns)}
@@ -14,8 +14,8 @@
 //         lds[(b - b mod ns) R + (b mod ns) + r ns] = y_r
 //     A thread reads all of its butterflies, the workgroup synchronises, then
 //     it computes and writes them back.
-//  3. Cooperative store with the fused store operations, ordered like the load
-//     by out_jfast.
+//  3. Store with the fused store operations, threads mapped like the load
+//     (by out_jfast).
 // The DFT computed is always the forward one (negative exponent); inverse
 // transforms conjugate on the way in and out.
 #pragma once
@@ -67,27 +67,51 @@ __device__ __forceinline__ void lds_stage(const KfaFftArgs& A, uint32_t s, cx<T>
         break;
 
 // One workgroup tile of a pass: instances [first, first + ipw).  The
-// workgroup may be larger than tpi * ipw (threads beyond it idle in the
-// stages); LDS must hold ipw * inst_stride elements.
+// workgroup may be larger than tpi * ipw (threads beyond it idle); LDS must
+// hold ipw * inst_stride elements.
 template <typename T, int MAXE>
 __device__ __forceinline__ void fft_tile(const KfaFftArgs& A, uint64_t first, cx<T>* lds) {
-    const uint32_t tid = threadIdx.x, nthr = blockDim.x;
+    const uint32_t tid = threadIdx.x;
     const uint32_t nin =
         (uint32_t)((A.batch - first) < (uint64_t)A.ipw ? (A.batch - first) : (uint64_t)A.ipw);
-    const uint32_t L = A.L, total = nin * L;
+    const uint32_t L = A.L;
 
-    // 1. cooperative load
-    for (uint32_t idx = tid; idx < total; idx += nthr) {
-        uint32_t j, e;
-        if (A.in_jfast) {
-            j = idx % nin;
-            e = idx / nin;
-        } else {
-            j = idx / L;
-            e = idx - j * L;
+    // 1. load.  Thread -> (instance, lane) as in the stages: lanes of one instance
+    // on consecutive threads (rows), or instances on consecutive threads (in_jfast,
+    // columns); each thread walks elements lane, lane + tpi, ...
+    const uint32_t tpi = A.tpi, ipw = A.ipw;
+    {
+        const uint32_t j = A.in_jfast ? tid % ipw : tid / tpi;
+        const uint32_t ln = A.in_jfast ? tid / ipw : tid % tpi;
+        if (ln < tpi && j < nin) {
+            const KfaIo& io = A.in;
+            const Ctx c = make_ctx(io, first + j);
+            cx<T>* dst = lds + (size_t)j * A.inst_stride;
+            const bool plain = !(io.flags & (KFA_IO_MAP | KFA_IO_PW)) && io.valid == 0xffffffffu;
+            if (plain) {
+                // address = base + (lgb + e * e_mul) * stride, advanced by a constant
+                const int64_t step = (int64_t)tpi * io.e_mul * io.stride;
+                int64_t a0 = c.base + ((int64_t)c.lgb + (int64_t)ln * io.e_mul) * io.stride;
+                const float sgn = (io.flags & (KFA_IO_CONJ1 | KFA_IO_CONJ2)) &&
+                                          !((io.flags & KFA_IO_CONJ1) && (io.flags & KFA_IO_CONJ2))
+                                      ? -1.0f : 1.0f;
+                if (io.flags & KFA_IO_REAL) {
+                    const T* src = (const T*)io.ptr;
+                    for (uint32_t e = ln; e < L; e += tpi, a0 += step)
+                        dst[lds_index<T>(e)] = {src[a0], T(0)};
+                } else {
+                    const cx<T>* src = (const cx<T>*)io.ptr;
+                    for (uint32_t e = ln; e < L; e += tpi, a0 += step) {
+                        cx<T> v = src[a0];
+                        v.im *= (T)sgn;
+                        dst[lds_index<T>(e)] = v;
+                    }
+                }
+            } else {
+                for (uint32_t e = ln; e < L; e += tpi)
+                    dst[lds_index<T>(e)] = load_elem<T>(io, c, e);
+            }
         }
-        const Ctx c = make_ctx(A.in, first + j);
-        lds[(size_t)j * A.inst_stride + lds_index<T>(e)] = load_elem<T>(A.in, c, e);
     }
     __syncthreads();
 
@@ -117,20 +141,18 @@ __device__ __forceinline__ void fft_tile(const KfaFftArgs& A, uint64_t first, cx
         }
     }
 
-    // 3. cooperative store
-    const bool addx0 = (A.out.flags & KFA_IO_ADDX0) != 0;
-    for (uint32_t idx = tid; idx < total; idx += nthr) {
-        uint32_t j, e;
-        if (A.out_jfast) {
-            j = idx % nin;
-            e = idx / nin;
-        } else {
-            j = idx / L;
-            e = idx - j * L;
+    // 3. store, thread mapping as the load's (out_jfast)
+    {
+        const bool addx0 = (A.out.flags & KFA_IO_ADDX0) != 0;
+        const uint32_t j = A.out_jfast ? tid % ipw : tid / tpi;
+        const uint32_t ln = A.out_jfast ? tid / ipw : tid % tpi;
+        if (ln < tpi && j < nin) {
+            const Ctx c = make_ctx(A.out, first + j);
+            const cx<T> x0 = addx0 ? load_aux<T>(A.out, c) : cx<T>{T(0), T(0)};
+            const cx<T>* srcl = lds + (size_t)j * A.inst_stride;
+            for (uint32_t e = ln; e < L; e += tpi)
+                store_elem<T>(A, c, e, srcl[lds_index<T>(e)], x0);
         }
-        const Ctx c = make_ctx(A.out, first + j);
-        const cx<T> x0 = addx0 ? load_aux<T>(A.out, c) : cx<T>{T(0), T(0)};
-        store_elem<T>(A, c, e, lds[(size_t)j * A.inst_stride + lds_index<T>(e)], x0);
     }
 }
 
