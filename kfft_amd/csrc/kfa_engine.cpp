@@ -968,11 +968,15 @@ const char* Plan1D::describe() const {
     return desc_.c_str();
 }
 
-bool Plan1D::supports_columns() const { return !sub_.four; }
+bool Plan1D::supports_columns() const {
+    // one-pass plans whose column tiles hold enough columns (>= 8) for coalesced
+    // rows of memory; longer columns are better served by tiled transposes
+    return !sub_.four && sub_.direct.pick_ipw(true) >= 8;
+}
 
 bool Plan1D::supports_plain_columns() const {
-    // one-pass plans, and Direct four-step plans on the shaped passes (column four-step)
-    return !sub_.four || (kind_ == Kind::Direct && sub_.tw4_full.p);
+    // as supports_columns, plus Direct four-step plans on the shaped passes (column four-step)
+    return supports_columns() || (kind_ == Kind::Direct && sub_.four && sub_.tw4_full.p);
 }
 
 size_t Plan1D::scratch_bytes(uint64_t batch) const {
