@@ -477,6 +477,13 @@ static kfft_return_t sclr_fwd_dev(kfft_plan_sclr* P, const void* din, void* dout
         return hip_ok(e, "scalar fwd") ? KFFT_RET_SUCCESS : KFFT_RET_GPU_FAIL;
     }
     const uint32_t h = n / 2;
+    if (!expand && pl.fuses_real_post() && !pl.shaped_rows()) {
+        // one LDS pass: Z = FFT_h(x viewed as h complex) rebuilt into X in its store
+        Layout o = contiguous(dout, out_dist);
+        o.real_post = g->st.p;
+        hipError_t e = pl.execute(contiguous(din, h), o, batch, false, 1.0, g->scratch.p, s);
+        return hip_ok(e, "scalar fwd") ? KFFT_RET_SUCCESS : KFFT_RET_GPU_FAIL;
+    }
     // Z = FFT_h(x viewed as h complex), written where X goes, then rebuilt in place
     hipError_t e = pl.execute(contiguous(din, h), contiguous(dout, out_dist), batch, false, 1.0,
                               g->scratch.p, s);

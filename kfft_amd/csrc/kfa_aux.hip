@@ -208,12 +208,14 @@ __global__ void k_accumulate(cx<float>* y, const cx<float>* d, uint64_t n) {
     }
 }
 
+// one thread per (instance, bin pair): idx = b * (h/2 + 1) + kk
 template <typename T>
 __global__ void k_real_post(const RealPostArgs a) {
-    const uint32_t kk = blockIdx.x * blockDim.x + threadIdx.x; // 0..h/2
-    const uint64_t b = blockIdx.y + (uint64_t)gridDim.y * blockIdx.z;
-    if (b < a.batch && kk <= a.h / 2)
-        real_post_one<T>(a, kk, b);
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t per = a.h / 2 + 1;
+    const uint64_t b = idx / per;
+    if (b < a.batch)
+        real_post_one<T>(a, (uint32_t)(idx - b * per), b);
 }
 
 template <typename T>
@@ -331,11 +333,15 @@ hipError_t launch_real_post(int prec, const void* Z, int64_t z_dist, void* X, in
     a.h = h;
     a.expand = (uint32_t)expand;
     a.conj_out = (uint32_t)conj_out;
-    const unsigned bs = 128;
+    const unsigned bs = 256;
+    const uint64_t pairs = (uint64_t)(h / 2 + 1) * batch; // one thread per bin pair
     if (Recorder* r = recorder())
-        return r->add(KFA_REC_REAL_POST, prec, 0, 0, blocks_for(h / 2 + 1, KFA_RAG_THREADS) * batch,
-                      blocks_for(h / 2 + 1, KFA_RAG_THREADS), &a, sizeof(a));
-    const dim3 g = batch_grid(blocks_for(h / 2 + 1, bs), batch);
+        return r->add(KFA_REC_REAL_POST, prec, 0, 0, (pairs + KFA_RAG_THREADS - 1) / KFA_RAG_THREADS,
+                      h / 2 + 1, &a, sizeof(a));
+    const uint64_t blocks = (pairs + bs - 1) / bs;
+    if (blocks > 0x7fffffffull)
+        return hipErrorInvalidValue;
+    const dim3 g((unsigned)blocks);
     if (prec == 8)
         hipLaunchKernelGGL(k_real_post<double>, g, dim3(bs), 0, s, a);
     else

@@ -67,12 +67,8 @@ __device__ __forceinline__ void real_post_one(const RealPostArgs& a, uint32_t kk
         return;
     }
     const uint32_t k = kk;
-    const cx<T> za = z[k], zb = z[h - k];
-    const cx<T> bb = {zb.re, -zb.im};
-    const cx<T> f1 = za + bb, f2 = za - bb;
-    const cx<T> t = cmul(f2, st[k - 1]);
-    const cx<T> xk = {(f1.re + t.re) * T(0.5), (f1.im + t.im) * T(0.5)};
-    const cx<T> xh = {(f1.re - t.re) * T(0.5), (t.im - f1.im) * T(0.5)};
+    cx<T> xk, xh;
+    rebuild_pair<T>(z[k], z[h - k], st[k - 1], xk, xh);
     put(k, xk);
     put(h - k, xh);
     if (a.expand) {

@@ -23,6 +23,9 @@
 #define KFA_IO_SCALE 0x0040u      // multiply by KfaFftArgs::scale (store)
 #define KFA_IO_TW4 0x0080u        // four-step twiddle W_N^(b_lo * e) (store)
 #define KFA_IO_ADDX0 0x0100u      // add the instance's aux value (Rader x[0]) after all ops (store)
+#define KFA_IO_RPOST 0x0200u      // real-input rebuild on store: Z = FFT_L of 2L reals read as L
+                                  // complex values; writes bins X[0..L] (pw: super twiddles
+                                  // st[i] = exp(-i pi ((i+1)/L + 1/2)), compute precision)
 #define KFA_IO_MAP 0x0800u        // element permutation map[logical]
 #define KFA_IO_PW_F64 0x1000u     // pw table is double precision complex
 #define KFA_IO_SCALE_POST 0x2000u // multiply by KfaFftArgs::scale_post after ADDX0
@@ -129,7 +132,7 @@ struct RealPostArgs { // real-input rebuild, X from Z = FFT_h(x as complex)
 struct RagRecord {
     uint32_t type;      // KFA_REC_*
     uint32_t tiles;     // workgroup tiles of this launch
-    uint32_t gx;        // REAL_POST: tiles along k (tile = b * gx + kx)
+    uint32_t gx;        // REAL_POST: bin pairs per instance (h/2 + 1)
     int32_t dep;        // record that must be complete first (same queue), -1 none
     uint32_t dep_tiles; // its tile count
     uint32_t pad;

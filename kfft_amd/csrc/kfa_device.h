@@ -324,5 +324,41 @@ __device__ __forceinline__ uint32_t lds_index(uint32_t e) {
     return e + (e >> (sizeof(T) == 8 ? KFA_PAD_SHIFT_F64 : KFA_PAD_SHIFT_F32));
 }
 
+// Real-input rebuild of one bin pair (0 < k <= h/2) from Z = FFT_h(x as complex):
+// f1 = Z[k] + conj(Z[h-k]), f2 = Z[k] - conj(Z[h-k]), t = f2 * st[k-1],
+// X[k] = (f1 + t) / 2, X[h-k] = conj(f1 - t) / 2 (kfft_scalar.c:174-216).
+template <typename T>
+__device__ __forceinline__ void rebuild_pair(cx<T> za, cx<T> zb, cx<T> w, cx<T>& xk, cx<T>& xh) {
+    const cx<T> bb = {zb.re, -zb.im};
+    const cx<T> f1 = za + bb, f2 = za - bb;
+    const cx<T> t = cmul(f2, w);
+    xk = {(f1.re + t.re) * T(0.5), (f1.im + t.im) * T(0.5)};
+    xh = {(f1.re - t.re) * T(0.5), (t.im - f1.im) * T(0.5)};
+}
+
+// Store of a pass carrying KFA_IO_RPOST: the instance's half-length spectrum Z
+// (L values) is in LDS at z; bins X[0..L] of the real-input transform go to the
+// output, lanes ln, ln + tpi, ... taking one pair (k, L-k) each.  X[0] and X[L]
+// come from Z[0] with zero imaginary parts.
+template <typename T>
+__device__ __forceinline__ void store_real_post(const KfaIo& io, const Ctx& c, const cx<T>* z,
+                                                uint32_t L, uint32_t ln, uint32_t tpi) {
+    cx<T>* x = (cx<T>*)io.ptr + c.base;
+    const int64_t xs = io.stride;
+    const cx<T>* st = (const cx<T>*)io.pw;
+    for (uint32_t k = ln; k <= L / 2; k += tpi) {
+        if (k == 0) {
+            const cx<T> dc = z[lds_index<T>(0)];
+            x[0] = {dc.re + dc.im, T(0)};
+            x[(int64_t)L * xs] = {dc.re - dc.im, T(0)};
+            continue;
+        }
+        cx<T> xk, xh;
+        rebuild_pair<T>(z[lds_index<T>(k)], z[lds_index<T>(L - k)], st[k - 1], xk, xh);
+        x[(int64_t)k * xs] = xk;
+        x[(int64_t)(L - k) * xs] = xh;
+    }
+}
+
 
 } // namespace kfa

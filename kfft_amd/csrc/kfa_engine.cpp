@@ -779,6 +779,10 @@ static KfaIo io_from(const Layout& l) {
         io.inst = (uint64_t)l.inst;
         io.flags |= KFA_IO_INST;
     }
+    if (l.real_post) {
+        io.pw = (uint64_t)l.real_post;
+        io.flags |= KFA_IO_RPOST;
+    }
     return io;
 }
 
@@ -989,6 +993,16 @@ size_t Plan1D::scratch_bytes(uint64_t batch) const {
     return elems * eb + 256;
 }
 
+size_t Plan1D::pass_inst_bytes() const {
+    const uint32_t s = sub_.four ? std::max(sub_.col.inst_stride, sub_.row.inst_stride)
+                                 : sub_.direct.inst_stride;
+    return (size_t)s * elem_bytes(prec_);
+}
+
+bool Plan1D::shaped_rows() const {
+    return kind_ == Kind::Direct && !sub_.four && fast_len(sub_.L) && sub_.maxr >= 16;
+}
+
 hipError_t Plan1D::execute(const Layout& in, const Layout& out, uint64_t batch, bool inverse,
                            double scale, void* scratch, hipStream_t s, const Fuse* fuse) const {
     if (batch == 0)
@@ -996,6 +1010,8 @@ hipError_t Plan1D::execute(const Layout& in, const Layout& out, uint64_t batch, 
     const size_t eb = elem_bytes(prec_);
     KfaIo load = io_from(in), store = io_from(out);
     const bool scaled = scale != 1.0;
+    if (out.real_post && (!fuses_real_post() || inverse || scaled))
+        return hipErrorInvalidValue;
     if (kind_ == Kind::Identity || kind_ == Kind::Direct) {
         if (fuse && fuse->pw_in) {
             load.pw = (uint64_t)fuse->pw_in;

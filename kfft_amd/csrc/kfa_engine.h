@@ -58,6 +58,10 @@ struct Layout {
     uint32_t C = 1;
     uint32_t flags = 0; // KFA_IO_REAL / KFA_IO_REAL_PAIRS
     const int64_t* inst = nullptr; // device table of instance offsets (replaces dist_hi)
+    // output side: super twiddles of a real-input rebuild fused into the store
+    // (the transform is the half-length complex FFT of a real-input forward; the
+    // output holds n/2+1 bins).  Only where Plan1D::fuses_real_post().
+    const void* real_post = nullptr;
 };
 
 // Extra operations a caller fuses into a transform (used by the real-FFT,
@@ -168,6 +172,14 @@ class Plan1D {
     // column layouts of plain complex data (no real / mapped / pointwise element operations)
     bool supports_plain_columns() const;
     bool two_pass() const { return sub_.four; }
+    // LDS bytes one instance of the plan's largest pass occupies
+    size_t pass_inst_bytes() const;
+    // one LDS pass over the whole transform: the real-input rebuild can be fused
+    // into its store (Layout::real_post)
+    bool fuses_real_post() const { return kind_ == Kind::Direct && !sub_.four; }
+    // outside the ragged executor, plain rows of this length run a shaped pass
+    // (kfa_fast_impl.h), which has no fused rebuild
+    bool shaped_rows() const;
     int async_error() const { return sub_.fused_error(); }
 
   private:
@@ -210,8 +222,15 @@ struct RaggedChoice {
     uint32_t ipw_cap = 0;
 };
 
+// Default LDS budget of an executor workgroup: the LDS that lets as many
+// workgroups stay resident per CU as the executor's registers do (three for
+// FP32, two for FP64).
+uint32_t ragged_lds_budget(int prec);
+
 struct RaggedOptions {
-    uint32_t lds_budget = 80 * 1024; // per workgroup, for choosing instances per tile
+    // LDS per workgroup that tiles and one-pass plans must fit (0: the default of
+    // the precision, ragged_lds_budget)
+    uint32_t lds_budget = 0;
     uint32_t ipw_cap = 0;            // largest instances per tile (0: as many as fit)
     uint32_t wgs_per_cu = 0;         // resident workgroups per CU (0: occupancy limit)
     uint32_t parts_per_queue = 4;    // split of the work of a queue into groups (balance)

@@ -15,7 +15,8 @@
 //     A thread reads all of its butterflies, the workgroup synchronises, then
 //     it computes and writes them back.
 //  3. Store with the fused store operations, threads mapped like the load
-//     (by out_jfast).
+//     (by out_jfast); or, for KFA_IO_RPOST, the real-input rebuild of bin
+//     pairs (k, L-k) read from LDS.
 // The DFT computed is always the forward one (negative exponent); inverse
 // transforms conjugate on the way in and out.
 #pragma once
@@ -148,10 +149,14 @@ __device__ __forceinline__ void fft_tile(const KfaFftArgs& A, uint64_t first, cx
         const uint32_t ln = A.out_jfast ? tid / ipw : tid % tpi;
         if (ln < tpi && j < nin) {
             const Ctx c = make_ctx(A.out, first + j);
-            const cx<T> x0 = addx0 ? load_aux<T>(A.out, c) : cx<T>{T(0), T(0)};
             const cx<T>* srcl = lds + (size_t)j * A.inst_stride;
-            for (uint32_t e = ln; e < L; e += tpi)
-                store_elem<T>(A, c, e, srcl[lds_index<T>(e)], x0);
+            if (A.out.flags & KFA_IO_RPOST) {
+                store_real_post<T>(A.out, c, srcl, L, ln, tpi);
+            } else {
+                const cx<T> x0 = addx0 ? load_aux<T>(A.out, c) : cx<T>{T(0), T(0)};
+                for (uint32_t e = ln; e < L; e += tpi)
+                    store_elem<T>(A, c, e, srcl[lds_index<T>(e)], x0);
+            }
         }
     }
 }

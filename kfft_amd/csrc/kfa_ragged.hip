@@ -19,8 +19,13 @@
 
 namespace kfa {
 
+// A task's latency (ticket, dependency wait, HBM load, LDS stages, store)
+// is hidden only by other resident workgroups: registers are held to what
+// three (FP32) or two (FP64) workgroups per CU need, which the LDS budget of
+// the plans also allows.
 template <typename T>
-__global__ void __launch_bounds__(KFA_RAG_THREADS) k_ragged(const RagArgs R) {
+__global__ void __launch_bounds__(KFA_RAG_THREADS)
+    __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2))) k_ragged(const RagArgs R) {
     extern __shared__ __align__(16) unsigned char kfa_smem[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(kfa_smem);
     __shared__ uint32_t sh_ticket;
@@ -71,11 +76,12 @@ __global__ void __launch_bounds__(KFA_RAG_THREADS) k_ragged(const RagArgs R) {
             break;
         }
         case KFA_REC_REAL_POST: {
+            // one thread per (instance, bin pair); rc.gx = h/2 + 1 pairs per instance
             const RealPostArgs& p = *reinterpret_cast<const RealPostArgs*>(a);
-            const uint32_t b = tile / rc.gx, kx = tile - b * rc.gx;
-            const uint32_t kk = kx * KFA_RAG_THREADS + tid;
-            if (kk <= p.h / 2)
-                real_post_one<T>(p, kk, b);
+            const uint64_t idx = (uint64_t)tile * KFA_RAG_THREADS + tid;
+            const uint64_t b = idx / rc.gx;
+            if (b < p.batch)
+                real_post_one<T>(p, (uint32_t)(idx - b * rc.gx), b);
             break;
         }
         default:

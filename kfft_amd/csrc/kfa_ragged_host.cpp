@@ -97,8 +97,12 @@ RaggedPlan::~RaggedPlan() {
         (void)hipStreamDestroy(cap_);
 }
 
+uint32_t ragged_lds_budget(int prec) { return prec == 8 ? 80u * 1024u : 52u * 1024u; }
+
 bool RaggedPlan::build(const std::vector<RaggedItem>& items, const RaggedOptions& opt) {
     opt_ = opt;
+    if (!opt_.lds_budget)
+        opt_.lds_budget = ragged_lds_budget(prec_);
     nq_ = xcd_queues();
     if (!nq_) {
         set_error("ragged executor: the workgroup-to-XCD mapping could not be established");
@@ -140,6 +144,14 @@ bool RaggedPlan::build(const std::vector<RaggedItem>& items, const RaggedOptions
         }
         if (!plan)
             return false;
+        if (!plan->two_pass() && plan->pass_inst_bytes() > opt_.lds_budget) {
+            // one instance would not fit the workgroup LDS budget: two passes keep
+            // every workgroup's LDS, and with it the executor's occupancy, bounded
+            PlanVariant v4 = ch.v;
+            v4.four = true;
+            if (std::shared_ptr<Plan1D> p4 = get_plan(plen, prec_, v4))
+                plan = p4;
+        }
         const uint32_t cap = ch.ipw_cap ? ch.ipw_cap : opt.ipw_cap;
         plans_.push_back(plan);
         const DevBuf* st = nullptr;
@@ -248,6 +260,9 @@ bool RaggedPlan::build(const std::vector<RaggedItem>& items, const RaggedOptions
                 e = g.plan->execute(in, out, cnt, inverse_, scale, scr, s);
             } else if (g.n % 2) {
                 in.flags = KFA_IO_REAL;
+                e = g.plan->execute(in, out, cnt, false, 1.0, scr, s);
+            } else if (g.plan->fuses_real_post()) {
+                out.real_post = g.st->p; // rebuild in the pass's store
                 e = g.plan->execute(in, out, cnt, false, 1.0, scr, s);
             } else {
                 e = g.plan->execute(in, out, cnt, false, 1.0, scr, s);
@@ -438,9 +453,8 @@ double tune_measure(uint32_t n, uint32_t kind, uint32_t count, int prec, const R
     }
     RaggedOptions opt;
     opt.choose = [c](uint32_t, uint32_t, uint32_t) { return c; };
-    // the occupancy of a mixed batch, whose largest pass sets the LDS of every
-    // workgroup (about 68 KiB: two workgroups per CU)
-    opt.lds_floor = 68 * 1024;
+    // the occupancy of a mixed batch, whose largest tiles fill the LDS budget
+    opt.lds_floor = ragged_lds_budget(prec);
     RaggedPlan rp(prec, false, true);
     DevBuf in, out;
     if (!rp.build(items, opt) || !in.alloc(in_stride * count) || !out.alloc(out_stride * count) ||

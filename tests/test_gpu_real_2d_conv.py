@@ -44,6 +44,19 @@ def test_real_forward(n, expand):
     assert rel(got, ref) <= tol(n)
 
 
+@pytest.mark.parametrize("n", [12, 30, 100, 2018, 4160])
+def test_real_forward_many_per_tile(n):
+    """Many short transforms per workgroup tile: the rebuild fused into the pass's
+    store (single-pass half lengths) and the separate rebuild (Rader half length
+    1009) both cover every instance."""
+    rng = np.random.default_rng(n + 13)
+    x = rng.uniform(-1, 1, (257, n))
+    got = K.ScalarPlan(n).forward(x)
+    ref = np.fft.rfft(x, axis=-1)
+    assert got.shape == ref.shape
+    assert rel(got, ref) <= tol(n)
+
+
 @pytest.mark.parametrize("n", REAL_N)
 @pytest.mark.parametrize("norm", [True, False])
 def test_real_inverse(n, norm):
