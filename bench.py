@@ -352,7 +352,7 @@ def other_configs(steps: int):
     return out
 
 
-def cpu_baseline(n: int, sample_batch: int):
+def cpu_baseline(n: int, sample_batch: int, seconds: float = 10.0):
     import numpy as np
 
     import oracle as O
@@ -367,16 +367,20 @@ def cpu_baseline(n: int, sample_batch: int):
     threads = min(threads, 16)
     p = O.Plan(n)
     p(x[:1], threads=threads)  # warm
-    best = None
-    for _ in range(3):
+    # repeated passes over the sample until about `seconds` of CPU work are done
+    times = []
+    t_all = time.perf_counter()
+    while len(times) < 3 or time.perf_counter() - t_all < seconds:
         t = time.perf_counter()
         p(x, threads=threads)
-        dt = time.perf_counter() - t
-        best = dt if best is None else min(best, dt)
-    gflops = 5 * n * math.log2(n) * sample_batch / best / 1e9
+        times.append(time.perf_counter() - t)
+    total = time.perf_counter() - t_all
+    med = sorted(times)[len(times) // 2]
+    gflops = 5 * n * math.log2(n) * sample_batch / med / 1e9
     return {"value": round(gflops, 3), "unit": "GFLOP/s", "cores": threads, "kind": "port",
-            "sample": f"oracle FP64 C2C N={n} x {sample_batch} transforms, best of 3 "
-                      f"({best:.2f} s), OpenMP over the batch"}
+            "sample": f"oracle FP64 C2C N={n} x {sample_batch} transforms, median of "
+                      f"{len(times)} passes ({med:.3f} s each, {total:.1f} s in all), "
+                      f"OpenMP over the batch"}
 
 
 def main():
