@@ -30,6 +30,24 @@ __device__ __forceinline__ void lds_stage(const KfaFftArgs& A, uint32_t s, cx<T>
     constexpr int NB = (MAXE / R) > 0 ? (MAXE / R) : 1;
     const uint32_t nb = A.L / R, ns = A.st[s].ns, tpi = A.tpi;
     cx<T> v[NB][R];
+    // the stage twiddles do not depend on LDS: in FP32 (radix <= 16) their
+    // loads are issued first so that their latency overlaps the LDS reads and
+    // the barrier (FP64 and the large prime radices would spill the registers
+    // they hold across the barrier)
+    constexpr bool early = sizeof(T) == 4 && R <= 16;
+    cx<T> w[NB][R - 1];
+    if (early && ns > 1) {
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const uint32_t b = lane + i * tpi;
+            if (act && b < nb) {
+                const cx<T>* tw = (const cx<T>*)A.tw + A.st[s].tw_off + (b % ns) * (R - 1);
+#pragma unroll
+                for (int r = 1; r < R; ++r)
+                    w[i][r - 1] = tw[r - 1];
+            }
+        }
+    }
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
         const uint32_t b = lane + i * tpi;
@@ -49,7 +67,7 @@ __device__ __forceinline__ void lds_stage(const KfaFftArgs& A, uint32_t s, cx<T>
                 const cx<T>* tw = (const cx<T>*)A.tw + A.st[s].tw_off + k * (R - 1);
 #pragma unroll
                 for (int r = 1; r < R; ++r)
-                    v[i][r] = cmul(v[i][r], tw[r - 1]);
+                    v[i][r] = cmul(v[i][r], early ? w[i][r - 1] : tw[r - 1]);
             }
             dft<T, R>(v[i]);
             const uint32_t base = (b - k) * R + k;
