@@ -45,7 +45,9 @@ EXPORTS = [
     "kfft_config_scalar", "kfft_eval_scalar", "kfft_evali_scalar",
     "kfft_eval_scalar_internal", "kfft_evali_scalar_internal",
     "kfft_math_hadamard_cpx", "kfft_math_hadamard_scalar", "kfft_math_transpose_cpx",
-    "kfft_math_transpose_scalar",
+    "kfft_math_transpose_scalar", "kfft_math_transpose_ip_cpx", "kfft_math_transpose_ip_scalar",
+    "kfft_math_magnitude", "kfft_math_magnitude_ip", "kfft_math_average",
+    "kfft_simd_analize", "kfft_simd_info",
     "kfft_shift_cpx", "kfft_shift_scalar",
     "kfft_config2_cpx", "kfft_eval2_cpx", "kfft_shift2_cpx",
     "kfft_config2_scalar", "kfft_eval2_scalar", "kfft_evali2_scalar", "kfft_shift2_scalar",
@@ -71,6 +73,11 @@ RAGGED_COMPLEX, RAGGED_REAL = 0, 1
 
 class KfftError(RuntimeError):
     pass
+
+
+class SimdInfo(C.Structure):
+    """kfft_simd_t (incs/kfft_types.h:100-105): architecture id, extension bits."""
+    _fields_ = [("arch", C.c_uint8), ("ext", C.c_uint32)]
 
 
 class _Info(C.Structure):
@@ -127,6 +134,14 @@ def load(half: bool = False):
         "kfft_eval2_conv_scalar": (ret, [vp, vp, vp, vp]),
         "kfft_math_hadamard_cpx": (None, [vp, vp, u32]),
         "kfft_math_transpose_cpx": (None, [vp, vp, u32, u32]),
+        "kfft_math_transpose_scalar": (None, [vp, vp, u32, u32]),
+        "kfft_math_transpose_ip_cpx": (None, [vp, u32, u32]),
+        "kfft_math_transpose_ip_scalar": (None, [vp, u32, u32]),
+        "kfft_math_magnitude": (None, [vp, vp, u32]),
+        "kfft_math_magnitude_ip": (None, [vp, u32]),
+        "kfft_math_average": (C.c_float if half else C.c_double, [vp, u32]),
+        "kfft_simd_analize": (SimdInfo, []),
+        "kfft_simd_info": (SimdInfo, []),
         "kfft_shift_cpx": (None, [vp, u32, C.c_bool, vp]),
         "kfft_shift_scalar": (None, [vp, u32, C.c_bool, vp]),
         "kfft_shift2_cpx": (None, [vp, vp, u32, u32, C.c_bool, vp]),

@@ -15,6 +15,7 @@
 #include <new>
 
 #include "../../include/kfft.h"
+#include "../../include/kfft_simd_check.h"
 #include "kfa_engine.h"
 #include "kfa_kernels.h"
 #include "kfa_mlp.h"
@@ -853,7 +854,7 @@ static void hadamard_dev(void* acc, const void* other, uint64_t n, hipStream_t s
 // ------------------------------------------------------------------ shift
 // fftshift / ifftshift of one row (kfft_shift.c:4-61): for even sizes the
 // halves swap; for odd sizes the rotation is by floor(n/2) (forward) or
-// ceil(n/2) (inverse).  Host buffers only; device buffers are round-tripped.
+// ceil(n/2) (inverse).  Host buffers.
 template <typename T>
 static void shift_host(T* buf, uint32_t size, bool inverse) {
     if (size < 2)
@@ -870,51 +871,88 @@ static void shift_host(T* buf, uint32_t size, bool inverse) {
     free(tmp);
 }
 
+// Device buffers: one 2D roll on the GPU (in place when every dimension is even
+// or 1, else a gather through `tmp`, or through a temporary when tmp is not
+// device memory), synchronous like the host path.
 template <typename T>
-static void shift_any(T* buf, uint32_t size, bool inverse) {
-    if (!is_device_ptr(buf)) {
-        shift_host(buf, size, inverse);
-        return;
+static void shift_dev(T* buf, T* tmp, uint32_t x, uint32_t y, bool inverse) {
+    hipStream_t s = entry_stream();
+    void* t = nullptr;
+    bool own = false;
+    if (!shift_in_place(x, y)) {
+        if (tmp && is_device_ptr(tmp)) {
+            t = tmp;
+        } else if (hip_ok(hipMalloc(&t, sizeof(T) * (size_t)x * y), "shift temporary")) {
+            own = true;
+        } else {
+            return;
+        }
     }
-    T* h = (T*)malloc(sizeof(T) * size);
-    if (!h)
-        return;
-    (void)hipMemcpy(h, buf, sizeof(T) * size, hipMemcpyDeviceToHost);
-    shift_host(h, size, inverse);
-    (void)hipMemcpy(buf, h, sizeof(T) * size, hipMemcpyHostToDevice);
-    free(h);
+    (void)hip_ok(launch_shift((int)sizeof(T), buf, t, x, y, inverse, s), "shift");
+    (void)hip_ok(hipStreamSynchronize(s), "shift sync");
+    if (own)
+        (void)hipFree(t);
 }
 
+template <typename T>
+static void shift_any(T* buf, uint32_t size, bool inverse) {
+    if (is_device_ptr(buf)) {
+        shift_dev(buf, (T*)nullptr, size, 1, inverse);
+        return;
+    }
+    shift_host(buf, size, inverse);
+}
 
 // 2D shift (ext/2d/kfft_cpx2.c:188-240): rows, then columns.
 template <typename T>
-static void shift2_any(T* buf, uint32_t x, uint32_t y, bool inverse) {
-    const size_t n = (size_t)x * y;
-    const bool dev = is_device_ptr(buf);
-    T* h = dev ? (T*)malloc(sizeof(T) * n) : buf;
-    if (!h)
+static void shift2_any(T* buf, T* ftmp, uint32_t x, uint32_t y, bool inverse) {
+    if (is_device_ptr(buf)) {
+        shift_dev(buf, ftmp, x, y, inverse);
         return;
-    if (dev)
-        (void)hipMemcpy(h, buf, sizeof(T) * n, hipMemcpyDeviceToHost);
+    }
     for (uint32_t r = 0; r < y; ++r)
-        shift_host(h + (size_t)r * x, x, inverse);
+        shift_host(buf + (size_t)r * x, x, inverse);
     T* col = (T*)malloc(sizeof(T) * y);
     if (col) {
         for (uint32_t c = 0; c < x; ++c) {
             for (uint32_t r = 0; r < y; ++r)
-                col[r] = h[(size_t)r * x + c];
+                col[r] = buf[(size_t)r * x + c];
             shift_host(col, y, inverse);
             for (uint32_t r = 0; r < y; ++r)
-                h[(size_t)r * x + c] = col[r];
+                buf[(size_t)r * x + c] = col[r];
         }
         free(col);
     }
-    if (dev) {
-        (void)hipMemcpy(buf, h, sizeof(T) * n, hipMemcpyHostToDevice);
-        free(h);
-    }
 }
 
+
+// In-place transposes (kfft_math.c:80-121): the result is the out-of-place
+// transpose written over the input; device buffers go through a temporary.
+template <typename T>
+static void transpose_ip_any(T* buf, uint32_t x, uint32_t y) {
+    if (x <= 1 || y <= 1)
+        return;
+    const size_t n = (size_t)x * y;
+    if (is_device_ptr(buf)) {
+        hipStream_t s = entry_stream();
+        void* t = nullptr;
+        if (!hip_ok(hipMalloc(&t, sizeof(T) * n), "transpose temporary"))
+            return;
+        if (hip_ok(launch_transpose((int)sizeof(T), buf, t, x, y, 1, s), "transpose"))
+            (void)hip_ok(hipMemcpyAsync(buf, t, sizeof(T) * n, hipMemcpyDeviceToDevice, s),
+                         "transpose copy");
+        (void)hip_ok(hipStreamSynchronize(s), "transpose sync");
+        (void)hipFree(t);
+        return;
+    }
+    T* t = (T*)malloc(sizeof(T) * n);
+    if (!t)
+        return;
+    memcpy(t, buf, sizeof(T) * n);
+    for (uint64_t k = 0; k < (uint64_t)n; k++)
+        buf[k] = t[(uint64_t)x * (k % y) + k / y];
+    free(t);
+}
 
 // ====================================================================== API
 extern "C" {
@@ -1289,6 +1327,58 @@ KFFT_API kfft_return_t kfft_eval2_conv_scalar(kfft_plan_s2cnv* plan, const kfft_
                                                                           : KFFT_RET_GPU_FAIL;
 }
 
+// ------------------------------------------------------------------- SIMD
+// simd/kfft_simd_check.c:132-175.  The host CPU's extensions, as cpuid reports
+// them; the build itself uses none (its transforms run on the GPU).
+KFFT_API kfft_simd_t kfft_simd_analize(void) {
+    kfft_simd_t S = {0, 0};
+#if (defined(__x86_64__) || defined(__i386__)) && !defined(__HIP_DEVICE_COMPILE__)
+    S.arch = sizeof(void*) == 8 ? HW_ARCH_X64 : HW_ARCH_X86;
+    __builtin_cpu_init();
+    uint32_t e = sizeof(void*) == 8 ? HW_x64 : 0u;
+    e |= __builtin_cpu_supports("mmx") ? HW_MMX : 0u;
+    e |= __builtin_cpu_supports("popcnt") ? HW_ABM : 0u;
+    e |= __builtin_cpu_supports("bmi") ? HW_BMI1 : 0u;
+    e |= __builtin_cpu_supports("bmi2") ? HW_BMI2 : 0u;
+    e |= __builtin_cpu_supports("sse") ? HW_SSE : 0u;
+    e |= __builtin_cpu_supports("sse2") ? HW_SSE2 : 0u;
+    e |= __builtin_cpu_supports("sse3") ? HW_SSE3 : 0u;
+    e |= __builtin_cpu_supports("ssse3") ? HW_SSSE3 : 0u;
+    e |= __builtin_cpu_supports("sse4.1") ? HW_SSE41 : 0u;
+    e |= __builtin_cpu_supports("sse4.2") ? HW_SSE42 : 0u;
+    e |= __builtin_cpu_supports("sse4a") ? HW_SSE4a : 0u;
+    e |= __builtin_cpu_supports("aes") ? HW_AES : 0u;
+    e |= __builtin_cpu_supports("avx") ? HW_AVX : 0u;
+    e |= __builtin_cpu_supports("xop") ? HW_XOP : 0u;
+    e |= __builtin_cpu_supports("fma") ? HW_FMA3 : 0u;
+    e |= __builtin_cpu_supports("fma4") ? HW_FMA4 : 0u;
+    e |= __builtin_cpu_supports("avx2") ? HW_AVX2 : 0u;
+    e |= __builtin_cpu_supports("avx512f") ? HW_AVX512_F : 0u;
+    e |= __builtin_cpu_supports("avx512cd") ? HW_AVX512_CD : 0u;
+    e |= __builtin_cpu_supports("avx512vl") ? HW_AVX512_VL : 0u;
+    e |= __builtin_cpu_supports("avx512bw") ? HW_AVX512_BW : 0u;
+    e |= __builtin_cpu_supports("avx512dq") ? HW_AVX512_DQ : 0u;
+    e |= __builtin_cpu_supports("avx512ifma") ? HW_AVX512_IFMA : 0u;
+    e |= __builtin_cpu_supports("avx512vbmi") ? HW_AVX512_VBMI : 0u;
+    S.ext = e;
+#elif defined(__aarch64__) || defined(__arm__)
+    S.arch = HW_ARCH_ARM;
+#endif
+    return S;
+}
+
+KFFT_API kfft_simd_t kfft_simd_info(void) {
+    kfft_simd_t S = {0, 0};
+#if defined(__x86_64__)
+    S.arch = HW_ARCH_X64;
+#elif defined(__i386__)
+    S.arch = HW_ARCH_X86;
+#elif defined(__aarch64__) || defined(__arm__)
+    S.arch = HW_ARCH_ARM;
+#endif
+    return S;
+}
+
 // ------------------------------------------------------------------- math
 KFFT_API void kfft_math_hadamard_cpx(kfft_cpx* Fout, const kfft_cpx* Fin, uint32_t size) {
     if (is_device_ptr(Fout) && is_device_ptr(Fin)) {
@@ -1340,6 +1430,79 @@ KFFT_API void kfft_math_transpose_scalar(const kfft_scalar* Fin, kfft_scalar* Fo
         Fout[n] = Fin[(uint64_t)x * (n % y) + n / y];
 }
 
+KFFT_API void kfft_math_transpose_ip_cpx(kfft_cpx* Fin, const uint32_t x, const uint32_t y) {
+    transpose_ip_any(Fin, x, y);
+}
+KFFT_API void kfft_math_transpose_ip_scalar(kfft_scalar* Fin, const uint32_t x, const uint32_t y) {
+    transpose_ip_any(Fin, x, y);
+}
+
+// kfft_math.c:123-130: Fout[i] = |Fin[i]|, with a correctly rounded square root
+// where the reference's default build bisects to an absolute 5e-4
+KFFT_API void kfft_math_magnitude(const kfft_cpx* Fin, kfft_scalar* Fout, uint32_t size) {
+    if (is_device_ptr(Fin) && is_device_ptr(Fout)) {
+        hipStream_t s = entry_stream();
+        (void)hip_ok(launch_magnitude(PREC, Fin, Fout, size, s), "magnitude");
+        (void)hip_ok(hipStreamSynchronize(s), "magnitude sync");
+        return;
+    }
+    for (uint32_t i = 0; i < size; i++)
+        Fout[i] = std::sqrt(Fin[i].r * Fin[i].r + Fin[i].i * Fin[i].i);
+}
+
+// kfft_math.c:132-138: the magnitudes overwrite the first `size` reals of the
+// buffer.  Reading element i before writing real i is what the reference's
+// loop order gives serially; the GPU computes all magnitudes before any write.
+KFFT_API void kfft_math_magnitude_ip(kfft_cpx* Fin, uint32_t size) {
+    if (is_device_ptr(Fin)) {
+        hipStream_t s = entry_stream();
+        void* t = nullptr;
+        if (!hip_ok(hipMalloc(&t, sizeof(kfft_scalar) * std::max<uint32_t>(size, 1)),
+                    "magnitude temporary"))
+            return;
+        if (hip_ok(launch_magnitude(PREC, Fin, t, size, s), "magnitude"))
+            (void)hip_ok(hipMemcpyAsync(Fin, t, sizeof(kfft_scalar) * size,
+                                        hipMemcpyDeviceToDevice, s), "magnitude copy");
+        (void)hip_ok(hipStreamSynchronize(s), "magnitude sync");
+        (void)hipFree(t);
+        return;
+    }
+    kfft_scalar* out = (kfft_scalar*)Fin;
+    for (uint32_t i = 0; i < size; i++) {
+        const kfft_cpx v = Fin[i];
+        out[i] = std::sqrt(v.r * v.r + v.i * v.i);
+    }
+}
+
+// kfft_math.c:140-148: the mean of `size` reals, summed in FP64 in a fixed
+// order (the reference's OpenMP loop adds into one shared variable without a
+// reduction, so its multi-threaded result is not defined; its serial result is
+// the plain sum in the build's precision)
+KFFT_API kfft_scalar kfft_math_average(const kfft_scalar* Fin, uint32_t size) {
+    if (!size)
+        return (kfft_scalar)0 / (kfft_scalar)0;
+    double acc = 0.0;
+    if (is_device_ptr(Fin)) {
+        hipStream_t s = entry_stream();
+        const uint32_t blocks = (uint32_t)std::min<uint64_t>(1024, (size + 255) / 256);
+        double* part = nullptr;
+        if (!hip_ok(hipMalloc((void**)&part, sizeof(double) * blocks), "average partials"))
+            return (kfft_scalar)0 / (kfft_scalar)0;
+        std::vector<double> h(blocks);
+        if (hip_ok(launch_partial_sum(PREC, Fin, size, part, blocks, s), "average") &&
+            hip_ok(hipMemcpyAsync(h.data(), part, sizeof(double) * blocks,
+                                  hipMemcpyDeviceToHost, s), "average copy") &&
+            hip_ok(hipStreamSynchronize(s), "average sync"))
+            for (double v : h)
+                acc += v;
+        (void)hipFree(part);
+    } else {
+        for (uint32_t i = 0; i < size; i++)
+            acc += (double)Fin[i];
+    }
+    return (kfft_scalar)(acc / (double)size);
+}
+
 KFFT_API void kfft_shift_cpx(kfft_cpx* buf, const uint32_t size, const bool is_inverse,
                              kfft_pool_t* mmgr) {
     (void)mmgr;
@@ -1353,15 +1516,13 @@ KFFT_API void kfft_shift_scalar(kfft_scalar* buf, const uint32_t size, const boo
 
 KFFT_API void kfft_shift2_cpx(kfft_cpx* buf, kfft_cpx* ftmp, const uint32_t sz_x,
                               const uint32_t sz_y, const bool is_inverse, kfft_pool_t* mmgr) {
-    (void)ftmp;
     (void)mmgr;
-    shift2_any(buf, sz_x, sz_y, is_inverse);
+    shift2_any(buf, ftmp, sz_x, sz_y, is_inverse);
 }
 KFFT_API void kfft_shift2_scalar(kfft_scalar* buf, kfft_scalar* ftmp, const uint32_t sz_x,
                                  const uint32_t sz_y, const bool is_inverse, kfft_pool_t* mmgr) {
-    (void)ftmp;
     (void)mmgr;
-    shift2_any(buf, sz_x, sz_y, is_inverse);
+    shift2_any(buf, ftmp, sz_x, sz_y, is_inverse);
 }
 
 // -------------------------------------------------------------------- GPU

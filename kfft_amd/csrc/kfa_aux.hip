@@ -387,6 +387,139 @@ hipError_t launch_transpose(int elem_bytes, const void* in, void* out, uint32_t 
     return hipGetLastError();
 }
 
+// fftshift of a y x x row-major matrix (one row when y = 1, kfft_shift.c:4-61,
+// ext/2d/kfft_cpx2.c:188-240) as a 2D roll by (rx, ry):
+//   out[r][c] = in[(r - ry) mod y][(c - rx) mod x]
+template <typename E>
+__global__ void k_shift_gather(const E* src, E* dst, uint32_t x, uint32_t y, uint32_t rx,
+                               uint32_t ry) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)x * y)
+        return;
+    const uint32_t r = (uint32_t)(i / x), c = (uint32_t)(i - (uint64_t)r * x);
+    const uint32_t sr = r >= ry ? r - ry : r + y - ry;
+    const uint32_t sc = c >= rx ? c - rx : c + x - rx;
+    dst[i] = src[(uint64_t)sr * x + sc];
+}
+
+// The same roll in place when every dimension is even or 1: a roll by half of
+// each even dimension pairs element i with p(i) (an involution); the thread of
+// the smaller index swaps the pair.
+template <typename E>
+__global__ void k_shift_swap(E* buf, uint32_t x, uint32_t y) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)x * y)
+        return;
+    const uint32_t hx = (x % 2 == 0) ? x / 2 : 0, hy = (y % 2 == 0) ? y / 2 : 0;
+    const uint32_t r = (uint32_t)(i / x), c = (uint32_t)(i - (uint64_t)r * x);
+    const uint32_t pr = r + hy < y ? r + hy : r + hy - y;
+    const uint32_t pc = c + hx < x ? c + hx : c + hx - x;
+    const uint64_t p = (uint64_t)pr * x + pc;
+    if (i < p) {
+        const E a = buf[i];
+        buf[i] = buf[p];
+        buf[p] = a;
+    }
+}
+
+template <typename E>
+static void shift_launch_t(void* buf, void* tmp, uint32_t x, uint32_t y, uint32_t rx, uint32_t ry,
+                           bool swap, hipStream_t s) {
+    const uint64_t n = (uint64_t)x * y;
+    const dim3 g((unsigned)((n + 255) / 256)), b(256);
+    if (swap)
+        hipLaunchKernelGGL(k_shift_swap<E>, g, b, 0, s, (E*)buf, x, y);
+    else
+        hipLaunchKernelGGL(k_shift_gather<E>, g, b, 0, s, (const E*)buf, (E*)tmp, x, y, rx, ry);
+}
+
+bool shift_in_place(uint32_t x, uint32_t y) {
+    return (x % 2 == 0 || x == 1) && (y % 2 == 0 || y == 1);
+}
+
+hipError_t launch_shift(int elem_bytes, void* buf, void* tmp, uint32_t x, uint32_t y, bool inverse,
+                        hipStream_t s) {
+    const uint64_t n = (uint64_t)x * y;
+    if (n < 2)
+        return hipSuccess;
+    if ((n + 255) / 256 > 0x7fffffffull)
+        return hipErrorInvalidValue;
+    // forward rolls by floor(d/2), inverse by ceil(d/2) (equal for even d)
+    const uint32_t rx = inverse ? x - x / 2 : x / 2, ry = inverse ? y - y / 2 : y / 2;
+    const bool swap = shift_in_place(x, y);
+    if (!swap && !tmp)
+        return hipErrorInvalidValue;
+    switch (elem_bytes) {
+    case 4:
+        shift_launch_t<uint32_t>(buf, tmp, x, y, rx % x, ry % y, swap, s);
+        break;
+    case 8:
+        shift_launch_t<uint64_t>(buf, tmp, x, y, rx % x, ry % y, swap, s);
+        break;
+    case 16:
+        shift_launch_t<cx<double>>(buf, tmp, x, y, rx % x, ry % y, swap, s);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && !swap)
+        e = hipMemcpyAsync(buf, tmp, n * (uint64_t)elem_bytes, hipMemcpyDeviceToDevice, s);
+    return e;
+}
+
+// |z| = sqrt(re^2 + im^2) (kfft_math.c:123-138 with a correctly rounded root)
+template <typename T>
+__global__ void k_magnitude(const cx<T>* in, T* out, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const cx<T> v = in[i];
+        out[i] = sqrt(v.re * v.re + v.im * v.im);
+    }
+}
+
+hipError_t launch_magnitude(int prec, const void* in, void* out, uint64_t n, hipStream_t s) {
+    if (!n)
+        return hipSuccess;
+    const dim3 g(blocks_for(n, 256)), b(256);
+    if (prec == 8)
+        hipLaunchKernelGGL(k_magnitude<double>, g, b, 0, s, (const cx<double>*)in, (double*)out, n);
+    else
+        hipLaunchKernelGGL(k_magnitude<float>, g, b, 0, s, (const cx<float>*)in, (float*)out, n);
+    return hipGetLastError();
+}
+
+// Partial sums in FP64: block k sums elements k*256 + t, stepping by the grid,
+// in a fixed order (deterministic), into partial[k].
+template <typename T>
+__global__ void __launch_bounds__(256) k_partial_sum(const T* in, uint64_t n, double* partial) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * 256)
+        acc += (double)in[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (unsigned w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        partial[blockIdx.x] = red[0];
+}
+
+hipError_t launch_partial_sum(int prec, const void* in, uint64_t n, double* partial, uint32_t blocks,
+                              hipStream_t s) {
+    if (prec == 8)
+        hipLaunchKernelGGL(k_partial_sum<double>, dim3(blocks), dim3(256), 0, s, (const double*)in,
+                           n, partial);
+    else
+        hipLaunchKernelGGL(k_partial_sum<float>, dim3(blocks), dim3(256), 0, s, (const float*)in, n,
+                           partial);
+    return hipGetLastError();
+}
+
 hipError_t launch_hadamard(int prec, void* acc, const void* other, uint64_t n, hipStream_t s) {
     if (!n)
         return hipSuccess;

@@ -69,6 +69,17 @@ hipError_t launch_real_pre(int prec, const void* X, int64_t xdh, int64_t xdl, in
 hipError_t launch_transpose(int elem_bytes, const void* in, void* out, uint32_t x, uint32_t y,
                             uint32_t nmat, hipStream_t s);
 hipError_t launch_hadamard(int prec, void* acc, const void* other, uint64_t n, hipStream_t s);
+// fftshift (forward) / ifftshift (inverse) of a y x x row-major matrix of
+// elem_bytes-sized elements in place; tmp (x*y elements) is needed unless
+// shift_in_place(x, y)
+bool shift_in_place(uint32_t x, uint32_t y);
+hipError_t launch_shift(int elem_bytes, void* buf, void* tmp, uint32_t x, uint32_t y, bool inverse,
+                        hipStream_t s);
+// out[i] = |in[i]| of n complex values of the precision
+hipError_t launch_magnitude(int prec, const void* in, void* out, uint64_t n, hipStream_t s);
+// per-block FP64 sums of n reals of the precision into partial[blocks]
+hipError_t launch_partial_sum(int prec, const void* in, uint64_t n, double* partial, uint32_t blocks,
+                              hipStream_t s);
 hipError_t launch_hadamard_real(int prec, void* acc, const void* other, uint64_t n,
                                 hipStream_t s);
 hipError_t launch_copy(int prec, const void* in, void* out, uint64_t n, int conj_, double scale,

@@ -151,3 +151,53 @@ def test_bench_radix16_stage_count():
     assert bench._radix16_stages("direct n=512 inner=512 one-pass") == 1  # 16, 8, 4
     assert bench._radix16_stages("direct n=32 inner=32 one-pass") == 0    # 8, 4
     assert bench._radix16_stages("rader n=1009 inner=1008 one-pass") == 0
+
+
+@pytest.mark.parametrize("half", [False, True])
+def test_math_helpers_host(half):
+    """kfft_math_transpose_ip_*, kfft_math_magnitude(_ip), kfft_math_average
+    (incs/kfft_math.h:303-350) on host buffers against the oracle's
+    restatements: the in-place transpose equals the reference's cycle walk,
+    magnitudes are within the reference's bisection root (5e-4 absolute) and
+    within rounding of the exact root, the average is the serial mean."""
+    import oracle.mathops as M
+
+    lib = K.load(half)
+    ft = np.float32 if half else np.float64
+    ct = np.complex64 if half else np.complex128
+    rng = np.random.default_rng(3)
+    for x, y in [(3, 5), (4, 4), (7, 2), (1, 6)]:
+        a = (rng.uniform(-1, 1, x * y) + 1j * rng.uniform(-1, 1, x * y)).astype(ct)
+        b = a.copy()
+        lib.kfft_math_transpose_ip_cpx(C.c_void_p(b.ctypes.data), x, y)
+        assert np.array_equal(b, M.transpose_ip(a, x, y))
+        r = a.real.copy()
+        lib.kfft_math_transpose_ip_scalar(C.c_void_p(r.ctypes.data), x, y)
+        assert np.array_equal(r, M.transpose_ip(a.real, x, y))
+    z = (rng.uniform(-3, 3, 257) + 1j * rng.uniform(-3, 3, 257)).astype(ct)
+    m = np.empty(257, ft)
+    lib.kfft_math_magnitude(C.c_void_p(z.ctypes.data), C.c_void_p(m.ctypes.data), 257)
+    exact = np.abs(z.astype(np.complex128))
+    assert np.max(np.abs(m - exact) / exact) <= (2e-7 if half else 1e-15)
+    assert np.max(np.abs(m - M.magnitude(z, ft))) <= 5e-4 + 1e-6
+    zi = z.copy()
+    lib.kfft_math_magnitude_ip(C.c_void_p(zi.ctypes.data), 257)
+    assert np.array_equal(zi.view(ft)[:257], m)
+    s = rng.uniform(-1, 1, 1001).astype(ft)
+    avg = lib.kfft_math_average(C.c_void_p(s.ctypes.data), 1001)
+    assert abs(avg - M.average_serial(s, np.float64)) <= (1e-7 if half else 1e-16)
+
+
+def test_simd_info():
+    """kfft_simd_info / kfft_simd_analize (simd/kfft_simd_check.h:58-66): the
+    build uses no CPU extensions (its transforms run on the GPU); the host
+    analysis reports an x86-64 CPU with at least its baseline SSE2."""
+    import platform
+
+    lib = K.load(False)
+    i = lib.kfft_simd_info()
+    a = lib.kfft_simd_analize()
+    if platform.machine() == "x86_64":
+        assert i.arch == 2 and i.ext == 0  # HW_ARCH_X64
+        assert a.arch == 2
+        assert a.ext & (1 << 1) and a.ext & (1 << 10)  # HW_x64, HW_SSE2

@@ -257,3 +257,23 @@ def test_conv2(y, x):
     else:
         with pytest.raises(RuntimeError):
             O.conv2(a + 0j, b + 0j)  # undefined in the reference (ext/conv2d/kfft_cpx_conv.c:67)
+
+
+def test_mathops_oracle_pins():
+    """The shift, in-place transpose and bisection-root restatements against
+    closed forms: numpy's fftshift / ifftshift, the matrix transpose, and the
+    bisection bracket bound (|err| <= 5e-4, the bracket is <= 1e-3 wide)."""
+    import oracle.mathops as M
+
+    for n in range(1, 14):
+        x = np.arange(n) + 0.5
+        assert np.array_equal(M.shift(x, False), np.fft.fftshift(x))
+        assert np.array_equal(M.shift(x, True), np.fft.ifftshift(x))
+    for x, y in [(3, 5), (4, 4), (7, 2), (2, 7), (1, 5), (6, 1), (5, 3)]:
+        a = np.arange(x * y) + 0.25
+        assert np.array_equal(M.transpose_ip(a, x, y), a.reshape(y, x).T.reshape(-1))
+    v = np.array([0.0, -1.0, 1e-9, 0.25, 1.0, 2.0, 9.0, 1234.5, 1e6])
+    r = M.sqrt_bisect(v)
+    assert r[0] == 0 and r[1] == 0
+    assert np.max(np.abs(r - np.sqrt(np.maximum(v, 0)))) <= 5e-4
+    assert M.average_serial([1.0, 2.0, 4.0]) == 7.0 / 3.0
