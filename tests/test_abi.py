@@ -201,3 +201,23 @@ def test_simd_info():
         assert i.arch == 2 and i.ext == 0  # HW_ARCH_X64
         assert a.arch == 2
         assert a.ext & (1 << 1) and a.ext & (1 << 10)  # HW_x64, HW_SSE2
+
+
+@pytest.mark.parametrize("lang", ["c", "c++"])
+def test_public_headers_compile(lang, tmp_path):
+    """kfft.h with the tiny API (kfft_tiny.h) and the SIMD-info header compile
+    as C and as C++ with warnings as errors."""
+    import shutil
+    import subprocess
+
+    cc = "gcc" if lang == "c" else "g++"
+    if not shutil.which(cc):
+        pytest.skip(f"{cc} not available")
+    src = tmp_path / ("t.c" if lang == "c" else "t.cpp")
+    src.write_text('#include "kfft_tiny.h"\n#include "kfft_simd_check.h"\n'
+                   "int main(void) { kfft_args_norm a = {16}; (void)a;\n"
+                   "  return (int)kfft_simd_align(kfft_simd_info()); }\n")
+    r = subprocess.run([cc, "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
+                        "-I" + os.path.join(ROOT, "include"), str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

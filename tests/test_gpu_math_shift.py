@@ -114,3 +114,21 @@ def test_magnitude_and_average_device(half):
     assert abs(avg - np.mean(s.astype(np.float64))) <= (1e-7 if half else 1e-15)
     assert lib.kfft_math_average(_p(_dev(s[:7])), 7) == pytest.approx(M.average_serial(s[:7]),
                                                                         abs=1e-7 if half else 1e-16)
+
+
+def test_tiny_api_program():
+    """tests/c/tiny_check.c (built by build()): the tiny API's one-handle plans
+    of every 1D / 2D / convolution kind through the C headers, checked in C
+    against brute-force DFTs and direct convolutions."""
+    import os
+    import subprocess
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "tests", "c", "build", "tiny_check")
+    if not os.path.exists(exe):
+        pytest.skip("tests/c programs not built (run __graft_entry__.build())")
+    print("running", exe, flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "tiny api: ok" in r.stdout
