@@ -51,8 +51,12 @@ struct KfaIo {
     uint32_t flags, pad0;
 };
 
+// One LDS stage: radix, ns (product of the earlier radices), offset of its
+// twiddles, and ns_magic = ceil(2^32 / ns) (0 for ns = 1): for b, ns < 2^16,
+// b / ns = umulhi(b, ns_magic) exactly (the rounding error times b stays
+// below 2^32), which replaces a runtime division per butterfly.
 struct KfaStage {
-    uint32_t radix, ns, tw_off, pad;
+    uint32_t radix, ns, tw_off, ns_magic;
 };
 
 struct KfaFftArgs {

@@ -186,6 +186,10 @@ bool lds_capable(uint32_t L, int prec) {
 bool LdsPass::plan(uint32_t len, int p, uint32_t max_radix) {
     L = len;
     prec = p;
+    if (L >= 65536u) { // the stages' multiply-high modulo needs b, ns < 2^16
+        set_error("length " + std::to_string(L) + " is too long for one LDS pass");
+        return false;
+    }
     if (!lds_shape(L, prec, &maxe, &tpi, &inst_stride, max_radix)) {
         set_error("length " + std::to_string(L) + " does not fit one LDS pass");
         return false;
@@ -197,6 +201,7 @@ bool LdsPass::plan(uint32_t len, int p, uint32_t max_radix) {
         st[s].radix = rl[s];
         st[s].ns = ns;
         st[s].tw_off = off;
+        st[s].ns_magic = ns > 1 ? (uint32_t)(((1ull << 32) + ns - 1) / ns) : 0u;
         if (s > 0)
             off += ns * (rl[s] - 1);
         ns *= rl[s];

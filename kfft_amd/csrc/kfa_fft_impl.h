@@ -28,7 +28,9 @@ template <typename T, int R, int MAXE>
 __device__ __forceinline__ void lds_stage(const KfaFftArgs& A, uint32_t s, cx<T>* my, uint32_t lane,
                                           bool act) {
     constexpr int NB = (MAXE / R) > 0 ? (MAXE / R) : 1;
-    const uint32_t nb = A.L / R, ns = A.st[s].ns, tpi = A.tpi;
+    const uint32_t nb = A.L / R, ns = A.st[s].ns, tpi = A.tpi, magic = A.st[s].ns_magic;
+    // b mod ns with b, ns < 2^16 (the LDS pass lengths): one multiply-high
+    auto mod_ns = [&](uint32_t b) { return b - __umulhi(b, magic) * ns; };
     cx<T> v[NB][R];
     // the stage twiddles do not depend on LDS: in FP32 (radix <= 16) their
     // loads are issued first so that their latency overlaps the LDS reads and
@@ -41,7 +43,7 @@ __device__ __forceinline__ void lds_stage(const KfaFftArgs& A, uint32_t s, cx<T>
         for (int i = 0; i < NB; ++i) {
             const uint32_t b = lane + i * tpi;
             if (act && b < nb) {
-                const cx<T>* tw = (const cx<T>*)A.tw + A.st[s].tw_off + (b % ns) * (R - 1);
+                const cx<T>* tw = (const cx<T>*)A.tw + A.st[s].tw_off + mod_ns(b) * (R - 1);
 #pragma unroll
                 for (int r = 1; r < R; ++r)
                     w[i][r - 1] = tw[r - 1];
@@ -62,7 +64,7 @@ __device__ __forceinline__ void lds_stage(const KfaFftArgs& A, uint32_t s, cx<T>
     for (int i = 0; i < NB; ++i) {
         const uint32_t b = lane + i * tpi;
         if (act && b < nb) {
-            const uint32_t k = (ns == 1) ? 0u : (b % ns);
+            const uint32_t k = (ns == 1) ? 0u : mod_ns(b);
             if (ns > 1) {
                 const cx<T>* tw = (const cx<T>*)A.tw + A.st[s].tw_off + k * (R - 1);
 #pragma unroll
