@@ -71,7 +71,8 @@ struct KfaFftArgs {
     uint32_t in_jfast;         // cooperative load: consecutive threads on consecutive instances
     uint32_t out_jfast;        // cooperative store: same for the store side
     uint32_t inst_stride;      // LDS elements per instance (padded, odd)
-    uint32_t pad1;
+    uint32_t tw_lds;           // FP64: stage twiddles copied into LDS after the instances at
+                               // the start of a tile (entries; 0: stages read them from memory)
     KfaStage st[KFA_MAX_STAGES];
 };
 

@@ -206,6 +206,7 @@ bool LdsPass::plan(uint32_t len, int p, uint32_t max_radix) {
             off += ns * (rl[s] - 1);
         ns *= rl[s];
     }
+    tw_count = nstages >= 2 ? off : 0u;
     if (nstages >= 2) {
         if (!tw.alloc((size_t)std::max<uint32_t>(off, 1) * elem_bytes(prec)))
             return false;
@@ -414,7 +415,12 @@ static hipError_t run_pass(const LdsPass& P, const KfaIo& in, const KfaIo& out, 
     a.ipw = P.pick_ipw(a.in_jfast || a.out_jfast);
     a.inst_stride = P.inst_stride;
     memcpy(a.st, P.st, sizeof(a.st));
-    return launch_fft(P.prec, P.maxe, a, P.lds_bytes(a.ipw), s);
+    // FP64: stage twiddles in LDS when they fit beside the instances within what
+    // keeps two workgroups per CU: each stage then reads them at LDS latency
+    // instead of waiting on a memory round trip (the FP32 passes prefetch them)
+    const size_t data = P.lds_bytes(a.ipw), twb = (size_t)P.tw_count * elem_bytes(P.prec);
+    a.tw_lds = (P.prec == 8 && P.tw_count && data + twb <= 80 * 1024) ? P.tw_count : 0u;
+    return launch_fft(P.prec, P.maxe, a, (uint32_t)(data + (a.tw_lds ? twb : 0)), s);
 }
 
 // Both passes of the fast two-pass transform for `batch` contiguous instances.

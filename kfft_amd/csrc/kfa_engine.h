@@ -79,6 +79,7 @@ struct LdsPass {
     int prec = 8; // bytes per real component
     int maxe = 16;
     uint32_t tpi = 0, nstages = 0, inst_stride = 0;
+    uint32_t tw_count = 0; // stage twiddle entries (sum over stages s >= 1 of ns (R - 1))
     KfaStage st[KFA_MAX_STAGES] = {};
     DevBuf tw; // stage twiddles
 

@@ -26,7 +26,7 @@ namespace kfa {
 
 template <typename T, int R, int MAXE>
 __device__ __forceinline__ void lds_stage(const KfaFftArgs& A, uint32_t s, cx<T>* my, uint32_t lane,
-                                          bool act) {
+                                          bool act, const cx<T>* ltw) {
     constexpr int NB = (MAXE / R) > 0 ? (MAXE / R) : 1;
     const uint32_t nb = A.L / R, ns = A.st[s].ns, tpi = A.tpi, magic = A.st[s].ns_magic;
     // b mod ns with b, ns < 2^16 (the LDS pass lengths): one multiply-high
@@ -37,8 +37,11 @@ __device__ __forceinline__ void lds_stage(const KfaFftArgs& A, uint32_t s, cx<T>
     // the barrier (FP64 and the large prime radices would spill the registers
     // they hold across the barrier)
     constexpr bool early = sizeof(T) == 4 && R <= 16;
+    // staged at the start of the tile (FP64 only: the FP32 passes are faster with
+    // the registers the second path would take, and prefetch instead)
+    const bool tw_in_lds = sizeof(T) == 8 && A.tw_lds != 0;
     cx<T> w[NB][R - 1];
-    if (early && ns > 1) {
+    if (early && !tw_in_lds && ns > 1) {
 #pragma unroll
         for (int i = 0; i < NB; ++i) {
             const uint32_t b = lane + i * tpi;
@@ -66,10 +69,17 @@ __device__ __forceinline__ void lds_stage(const KfaFftArgs& A, uint32_t s, cx<T>
         if (act && b < nb) {
             const uint32_t k = (ns == 1) ? 0u : mod_ns(b);
             if (ns > 1) {
-                const cx<T>* tw = (const cx<T>*)A.tw + A.st[s].tw_off + k * (R - 1);
+                if (tw_in_lds) {
+                    const cx<T>* tw = ltw + A.st[s].tw_off + k * (R - 1);
 #pragma unroll
-                for (int r = 1; r < R; ++r)
-                    v[i][r] = cmul(v[i][r], early ? w[i][r - 1] : tw[r - 1]);
+                    for (int r = 1; r < R; ++r)
+                        v[i][r] = cmul(v[i][r], tw[r - 1]);
+                } else {
+                    const cx<T>* tw = (const cx<T>*)A.tw + A.st[s].tw_off + k * (R - 1);
+#pragma unroll
+                    for (int r = 1; r < R; ++r)
+                        v[i][r] = cmul(v[i][r], early ? w[i][r - 1] : tw[r - 1]);
+                }
             }
             dft<T, R>(v[i]);
             const uint32_t base = (b - k) * R + k;
@@ -84,12 +94,12 @@ __device__ __forceinline__ void lds_stage(const KfaFftArgs& A, uint32_t s, cx<T>
 #define KFA_STAGE_CASE(R)                                                                         \
     case R:                                                                                       \
         if constexpr (R <= MAXE)                                                                  \
-            lds_stage<T, R, MAXE>(A, s, my, lane, act);                                           \
+            lds_stage<T, R, MAXE>(A, s, my, lane, act, ltw);                                      \
         break;
 
 // One workgroup tile of a pass: instances [first, first + ipw).  The
 // workgroup may be larger than tpi * ipw (threads beyond it idle); LDS must
-// hold ipw * inst_stride elements.
+// hold ipw * inst_stride elements, then tw_lds stage twiddles (FP64).
 template <typename T, int MAXE>
 __device__ __forceinline__ void fft_tile(const KfaFftArgs& A, uint64_t first, cx<T>* lds) {
     const uint32_t tid = threadIdx.x;
@@ -133,6 +143,13 @@ __device__ __forceinline__ void fft_tile(const KfaFftArgs& A, uint64_t first, cx
                     dst[lds_index<T>(e)] = load_elem<T>(io, c, e);
             }
         }
+    }
+    // the stage twiddles, when staged, behind the instances
+    cx<T>* ltw = lds + (size_t)ipw * A.inst_stride;
+    if (sizeof(T) == 8 && A.tw_lds) {
+        const cx<T>* g = (const cx<T>*)A.tw;
+        for (uint32_t i = tid; i < A.tw_lds; i += blockDim.x)
+            ltw[i] = g[i];
     }
     __syncthreads();
 

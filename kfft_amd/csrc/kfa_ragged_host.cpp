@@ -83,6 +83,16 @@ bool rag_shape(KfaFftArgs& A, int prec, const RaggedOptions& o, uint32_t cap, ui
     A.tpi = tpi;
     A.ipw = ipw;
     *lds = (uint32_t)(per_inst * ipw);
+    // FP64: stage twiddles in LDS when they fit within the budget beside the instances
+    uint32_t twn = 0;
+    if (prec == 8 && A.nstages >= 2) {
+        const KfaStage& last = A.st[A.nstages - 1];
+        twn = last.tw_off + last.ns * (last.radix - 1);
+    }
+    const size_t twb = (size_t)twn * elem_bytes(prec);
+    A.tw_lds = (twn && *lds + twb <= o.lds_budget) ? twn : 0u;
+    if (A.tw_lds)
+        *lds += (uint32_t)twb;
     return true;
 }
 
