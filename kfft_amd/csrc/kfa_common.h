@@ -66,6 +66,11 @@ struct KfaFftArgs {
     double scale, scale_post;
     uint64_t batch;            // number of instances
     uint32_t tw4_n, tw4_shift; // four-step length and log2 of the low table size
+    // convolution pass (Bluestein): after the stages, element e of instance b is
+    // multiplied by mid_pw[(b mod mid_rows) * L + e] (compute precision) and
+    // conjugated, and the stages run again (0: an ordinary pass)
+    uint64_t mid_pw;
+    uint32_t mid_rows, pad2;
     uint32_t L, nstages;
     uint32_t tpi, ipw;         // threads per instance, instances per workgroup
     uint32_t in_jfast;         // cooperative load: consecutive threads on consecutive instances

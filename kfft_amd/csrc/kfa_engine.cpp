@@ -394,9 +394,12 @@ static bool io_jfast(const KfaIo& io) {
 
 static hipError_t run_pass(const LdsPass& P, const KfaIo& in, const KfaIo& out, double scale,
                            double scale_post, uint64_t batch, const void* tw4_hi,
-                           const void* tw4_lo, uint32_t tw4_n, uint32_t tw4_shift, hipStream_t s) {
+                           const void* tw4_lo, uint32_t tw4_n, uint32_t tw4_shift, hipStream_t s,
+                           const void* mid_pw = nullptr, uint32_t mid_rows = 1) {
     KfaFftArgs a;
     memset(&a, 0, sizeof(a));
+    a.mid_pw = (uint64_t)mid_pw;
+    a.mid_rows = mid_rows ? mid_rows : 1u;
     a.in = in;
     a.out = out;
     a.tw = (uint64_t)P.tw.p;
@@ -512,6 +515,70 @@ hipError_t SubFft::run_four_cols(const KfaIo& load, const KfaIo& store, double s
             return e;
     }
     return hipSuccess;
+}
+
+// Bluestein's convolution y = IDFT(DFT(x) * h) (unnormalised) with the load's
+// element operations on x and the store's on y (whose CONJ1 finishes the
+// inverse).  One LDS pass: both transforms in one workgroup tile, h applied in
+// LDS between them.  Four-step L = N1 x N2 (C == 1 layouts): three passes over
+// `scratch` (batch x L):
+//   A1  columns n2: DFT over n1, times W_L^(n2 k1), into scratch[k1][n2]
+//   M   rows k1 in place: DFT over n2 (spectrum index k1 + N1 k2), times
+//       h[k1 + N1 k2] (hperm, laid out [k1][k2]), conjugate, DFT over k2 again,
+//       times W_L^(k1 n2), conjugate: the row step of the inverse transform,
+//       W_L^-(n2 + N2 n1)(k1 + N1 k2) = W_L^-(n2 k1) W_N2^-(n2 k2) W_N1^-(n1 k1)
+//   A2  columns n2: conjugate, DFT over k1, store element n2 + N2 n1
+// against four for two separate transforms.
+hipError_t SubFft::run_conv(const KfaIo& load, const KfaIo& store, double scale, uint64_t batch,
+                            const void* h, const void* hperm, void* scratch, hipStream_t s) const {
+    if (!four)
+        return run_pass(direct, load, store, scale, 1.0, batch, nullptr, nullptr, 0, 0, s, h, 1);
+    if (load.C != 1 || store.C != 1 || !hperm)
+        return hipErrorInvalidValue;
+    // A1: as pass A of the two-pass transform
+    KfaIo ia = load;
+    ia.C = N2;
+    ia.dist_lo = 0;
+    ia.lo_mul = 1;
+    ia.e_mul = N2;
+    ia.stride = load.stride;
+    KfaIo oa;
+    memset(&oa, 0, sizeof(oa));
+    oa.ptr = (uint64_t)scratch;
+    oa.C = N2;
+    oa.dist_hi = L;
+    oa.dist_lo = 1;
+    oa.stride = N2;
+    oa.e_mul = 1;
+    oa.valid = 0xffffffffu;
+    oa.flags = KFA_IO_TW4;
+    hipError_t e = run_pass(col, ia, oa, 1.0, 1.0, batch * N2, tw4_hi.p, tw4_lo.p, L, tw4_shift, s);
+    if (e != hipSuccess)
+        return e;
+    // M: rows k1 of the scratch, in place
+    KfaIo im;
+    memset(&im, 0, sizeof(im));
+    im.ptr = (uint64_t)scratch;
+    im.C = N1;
+    im.dist_hi = L;
+    im.dist_lo = N2;
+    im.stride = 1;
+    im.e_mul = 1;
+    im.valid = 0xffffffffu;
+    KfaIo om = im;
+    om.flags = KFA_IO_TW4 | KFA_IO_CONJ2;
+    e = run_pass(row, im, om, 1.0, 1.0, batch * N1, tw4_hi.p, tw4_lo.p, L, tw4_shift, s, hperm, N1);
+    if (e != hipSuccess)
+        return e;
+    // A2: columns n2 of the scratch, conjugated, DFT over k1, stored at n2 + N2 n1
+    KfaIo i2 = oa;
+    i2.flags = KFA_IO_CONJ1;
+    KfaIo o2 = store;
+    o2.C = N2;
+    o2.dist_lo = 0;
+    o2.lo_mul = 1;
+    o2.e_mul = N2;
+    return run_pass(col, i2, o2, scale, 1.0, batch * N2, nullptr, nullptr, 0, 0, s);
 }
 
 // ============================================================ fused two-pass
@@ -892,7 +959,23 @@ bool Plan1D::build() {
         return false;
     if (!hip_ok(hipStreamSynchronize(s), "bluestein sync"))
         return false;
-    return sub_.plan(M_, prec_, var_.four, var_.maxr);
+    if (!sub_.plan(M_, prec_, var_.four, var_.maxr))
+        return false;
+    if (sub_.four) {
+        // the convolution's middle pass reads h in the row order of its spectrum
+        const size_t eb = elem_bytes(prec_);
+        std::vector<unsigned char> a((size_t)M_ * eb), b((size_t)M_ * eb);
+        if (!hip_ok(hipMemcpy(a.data(), hhat_.p, a.size(), hipMemcpyDeviceToHost), "hhat") ||
+            !hperm_.alloc(b.size()))
+            return false;
+        const uint32_t N1 = sub_.N1, N2 = sub_.N2;
+        for (uint32_t k1 = 0; k1 < N1; ++k1)
+            for (uint32_t k2 = 0; k2 < N2; ++k2)
+                memcpy(&b[((size_t)k1 * N2 + k2) * eb], &a[((size_t)k1 + (size_t)N1 * k2) * eb], eb);
+        if (!hip_ok(hipMemcpy(hperm_.p, b.data(), b.size(), hipMemcpyHostToDevice), "hperm"))
+            return false;
+    }
+    return true;
 }
 
 void cost_features(uint32_t n, uint32_t kind, uint32_t count, int prec, const RaggedChoice& c,
@@ -1000,7 +1083,7 @@ size_t Plan1D::scratch_bytes(uint64_t batch) const {
     if (kind_ == Kind::Rader)
         elems += batch + (size_t)batch * sub_.L;
     else if (kind_ == Kind::Bluestein)
-        elems += (size_t)batch * sub_.L;
+        elems = (size_t)batch * sub_.L; // the convolution's four-step scratch
     return elems * eb + 256;
 }
 
@@ -1083,26 +1166,17 @@ hipError_t Plan1D::execute(const Layout& in, const Layout& out, uint64_t batch, 
         return launch_rader_dc(prec_, x0, real_in, tmpA, L, out.ptr, out.dist_hi, out.dist_lo,
                                out.C ? out.C : 1, real_out, inverse, scale, batch, s, out.inst);
     }
-    // Bluestein
+    // Bluestein: X = chirp . IDFT_M(DFT_M(chirp . x, zero padded) * FFT(h)) / M
     void* tmp = sp;
-    sp += (size_t)batch * L * eb;
-    void* sub_scr = sp;
     KfaIo l1 = load;
     l1.flags |= KFA_IO_PW | (inverse ? KFA_IO_CONJ1 : 0);
     l1.pw = (uint64_t)chirp_.p;
     l1.valid = n_;
-    KfaIo s1 = io_plain(tmp, L);
-    hipError_t e = sub_.run(l1, s1, 1.0, 1.0, batch, sub_scr, s, false, dft);
-    if (e != hipSuccess)
-        return e;
-    KfaIo l2 = io_plain(tmp, L);
-    l2.flags |= KFA_IO_PW | KFA_IO_CONJ2;
-    l2.pw = (uint64_t)hhat_.p;
     KfaIo s2 = store;
     s2.flags |= KFA_IO_CONJ1 | KFA_IO_PW | KFA_IO_SCALE | (inverse ? KFA_IO_CONJ2 : 0);
     s2.pw = (uint64_t)chirp_.p;
     s2.valid = n_;
-    return sub_.run(l2, s2, scale / (double)L, 1.0, batch, sub_scr, s, false, dft);
+    return sub_.run_conv(l1, s2, scale / (double)L, batch, hhat_.p, hperm_.p, tmp, s);
 }
 
 // =============================================================== plan registry

@@ -117,6 +117,12 @@ struct SubFft {
                    uint32_t dft = 0) const;
     hipError_t run_four_fast(const KfaIo& load, const KfaIo& store, double scale, uint64_t batch,
                              void* scratch, uint32_t nt, hipStream_t s, uint32_t dft) const;
+    // y = IDFT(DFT(x) * h), unnormalised, with the load's / store's element
+    // operations (the store's CONJ1 completes the inverse): one pass, or three
+    // for four-step lengths, where hperm holds h[k1 + N1 k2] at [k1][k2] and
+    // scratch holds batch x L values
+    hipError_t run_conv(const KfaIo& load, const KfaIo& store, double scale, uint64_t batch,
+                        const void* h, const void* hperm, void* scratch, hipStream_t s) const;
     // both passes on columns of row-major matrices (C = columns, dist_lo = 1): the
     // shaped column passes walk the columns with the lanes
     hipError_t run_four_cols(const KfaIo& load, const KfaIo& store, double scale, uint64_t batch,
@@ -192,6 +198,7 @@ class Plan1D {
     uint32_t M_ = 0;
     DevBuf qidx_, pidx_, bhat_; // Rader
     DevBuf chirp_, hhat_;       // Bluestein
+    DevBuf hperm_;              // Bluestein, four-step M: hhat[k1 + N1 k2] at [k1][k2]
     mutable std::string desc_;
 };
 

@@ -18,7 +18,9 @@
 //     (by out_jfast); or, for KFA_IO_RPOST, the real-input rebuild of bin
 //     pairs (k, L-k) read from LDS.
 // The DFT computed is always the forward one (negative exponent); inverse
-// transforms conjugate on the way in and out.
+// transforms conjugate on the way in and out.  A convolution pass (Bluestein,
+// mid_pw) multiplies the spectrum by a table and conjugates it between two
+// rounds of the stages, so its store sees conj(DFT(conj(DFT(x) * h))).
 #pragma once
 #include "kfa_device.h"
 
@@ -153,10 +155,25 @@ __device__ __forceinline__ void fft_tile(const KfaFftArgs& A, uint64_t first, cx
     }
     __syncthreads();
 
-    // 2. radix stages
+    // 2. radix stages; a convolution pass (mid_pw) runs them twice around a
+    // pointwise product and conjugation of the first round's spectrum
     const uint32_t il = tid / A.tpi, lane = tid - il * A.tpi;
     const bool act = il < nin;
     cx<T>* my = lds + (size_t)(act ? il : 0) * A.inst_stride;
+    const uint32_t rounds = A.mid_pw ? 2u : 1u;
+    for (uint32_t round = 0; round < rounds; ++round) {
+    if (round == 1) {
+        if (act) {
+            const uint64_t row = (first + il) % A.mid_rows;
+            const cx<T>* h = (const cx<T>*)A.mid_pw + row * L;
+            for (uint32_t e = lane; e < L; e += tpi) {
+                cx<T> v = cmul(my[lds_index<T>(e)], h[e]);
+                v.im = -v.im;
+                my[lds_index<T>(e)] = v;
+            }
+        }
+        __syncthreads();
+    }
     for (uint32_t s = 0; s < A.nstages; ++s) {
         switch (A.st[s].radix) {
             KFA_STAGE_CASE(2)
@@ -177,6 +194,7 @@ __device__ __forceinline__ void fft_tile(const KfaFftArgs& A, uint64_t first, cx
         default:
             break;
         }
+    }
     }
 
     // 3. store, thread mapping as the load's (out_jfast)

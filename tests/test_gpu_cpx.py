@@ -35,7 +35,7 @@ def cplx(rng, *shape):
 # wrongly (see tests/test_oracle.py), so those compare with numpy instead
 LENGTHS = [1, 2, 3, 4, 5, 7, 8, 9, 11, 13, 16, 17, 25, 31, 32, 37, 60, 64, 97, 100, 128, 210, 256,
            330, 512, 1000, 1008, 1009, 1024, 2048, 2018, 4096, 4160, 6561, 8192, 10000, 12289,
-           16384, 65536, 65537, 94132]
+           16384, 65536, 65537, 94132, 74, 1234, 30011]
 
 
 @pytest.mark.parametrize("n", LENGTHS)
