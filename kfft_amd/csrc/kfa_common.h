@@ -66,10 +66,13 @@ struct KfaFftArgs {
     double scale, scale_post;
     uint64_t batch;            // number of instances
     uint32_t tw4_n, tw4_shift; // four-step length and log2 of the low table size
-    // convolution pass (Bluestein): after the stages, element e of instance b is
-    // multiplied by mid_pw[(b mod mid_rows) * L + e] (compute precision) and
-    // conjugated, and the stages run again (0: an ordinary pass)
+    // convolution pass (Bluestein, Rader): after the stages, element e of
+    // instance b is multiplied by mid_pw[(b mod mid_rows) * L + e] (compute
+    // precision) and conjugated, and the stages run again (0: an ordinary pass).
+    // mid_dc (if set) receives the first round's element 0 of the instances with
+    // b mod mid_rows == 0, at index b / mid_rows (Rader's A[0]).
     uint64_t mid_pw;
+    uint64_t mid_dc;
     uint32_t mid_rows, pad2;
     uint32_t L, nstages;
     uint32_t tpi, ipw;         // threads per instance, instances per workgroup

@@ -395,10 +395,12 @@ static bool io_jfast(const KfaIo& io) {
 static hipError_t run_pass(const LdsPass& P, const KfaIo& in, const KfaIo& out, double scale,
                            double scale_post, uint64_t batch, const void* tw4_hi,
                            const void* tw4_lo, uint32_t tw4_n, uint32_t tw4_shift, hipStream_t s,
-                           const void* mid_pw = nullptr, uint32_t mid_rows = 1) {
+                           const void* mid_pw = nullptr, uint32_t mid_rows = 1,
+                           void* mid_dc = nullptr) {
     KfaFftArgs a;
     memset(&a, 0, sizeof(a));
     a.mid_pw = (uint64_t)mid_pw;
+    a.mid_dc = (uint64_t)mid_dc;
     a.mid_rows = mid_rows ? mid_rows : 1u;
     a.in = in;
     a.out = out;
@@ -528,11 +530,14 @@ hipError_t SubFft::run_four_cols(const KfaIo& load, const KfaIo& store, double s
 //       times W_L^(k1 n2), conjugate: the row step of the inverse transform,
 //       W_L^-(n2 + N2 n1)(k1 + N1 k2) = W_L^-(n2 k1) W_N2^-(n2 k2) W_N1^-(n1 k1)
 //   A2  columns n2: conjugate, DFT over k1, store element n2 + N2 n1
-// against four for two separate transforms.
-hipError_t SubFft::run_conv(const KfaIo& load, const KfaIo& store, double scale, uint64_t batch,
-                            const void* h, const void* hperm, void* scratch, hipStream_t s) const {
+// against four for two separate transforms.  `dc` (optional) receives the
+// spectrum's element 0 per instance (Rader's A[0]).
+hipError_t SubFft::run_conv(const KfaIo& load, const KfaIo& store, double scale,
+                            double scale_post, uint64_t batch, const void* h, const void* hperm,
+                            void* scratch, hipStream_t s, void* dc) const {
     if (!four)
-        return run_pass(direct, load, store, scale, 1.0, batch, nullptr, nullptr, 0, 0, s, h, 1);
+        return run_pass(direct, load, store, scale, scale_post, batch, nullptr, nullptr, 0, 0, s, h,
+                        1, dc);
     if (load.C != 1 || store.C != 1 || !hperm)
         return hipErrorInvalidValue;
     // A1: as pass A of the two-pass transform
@@ -567,7 +572,8 @@ hipError_t SubFft::run_conv(const KfaIo& load, const KfaIo& store, double scale,
     im.valid = 0xffffffffu;
     KfaIo om = im;
     om.flags = KFA_IO_TW4 | KFA_IO_CONJ2;
-    e = run_pass(row, im, om, 1.0, 1.0, batch * N1, tw4_hi.p, tw4_lo.p, L, tw4_shift, s, hperm, N1);
+    e = run_pass(row, im, om, 1.0, 1.0, batch * N1, tw4_hi.p, tw4_lo.p, L, tw4_shift, s, hperm, N1,
+                 dc);
     if (e != hipSuccess)
         return e;
     // A2: columns n2 of the scratch, conjugated, DFT over k1, stored at n2 + N2 n1
@@ -578,7 +584,7 @@ hipError_t SubFft::run_conv(const KfaIo& load, const KfaIo& store, double scale,
     o2.dist_lo = 0;
     o2.lo_mul = 1;
     o2.e_mul = N2;
-    return run_pass(col, i2, o2, scale, 1.0, batch * N2, nullptr, nullptr, 0, 0, s);
+    return run_pass(col, i2, o2, scale, scale_post, batch * N2, nullptr, nullptr, 0, 0, s);
 }
 
 // ============================================================ fused two-pass
@@ -896,6 +902,23 @@ static uint32_t smooth_at_least(uint64_t n, int prec) {
     return 0;
 }
 
+// For a four-step inner transform, the convolution's middle pass reads the
+// spectrum table h in its row order: out[k1 * N2 + k2] = h[k1 + N1 k2].
+static bool conv_table_rows(const DevBuf& h, const SubFft& sub, DevBuf& out) {
+    if (!sub.four)
+        return true;
+    const size_t eb = elem_bytes(sub.prec), n = (size_t)sub.L;
+    std::vector<unsigned char> a(n * eb), b(n * eb);
+    if (!hip_ok(hipMemcpy(a.data(), h.p, a.size(), hipMemcpyDeviceToHost), "conv table") ||
+        !out.alloc(b.size()))
+        return false;
+    const uint32_t N1 = sub.N1, N2 = sub.N2;
+    for (uint32_t k1 = 0; k1 < N1; ++k1)
+        for (uint32_t k2 = 0; k2 < N2; ++k2)
+            memcpy(&b[((size_t)k1 * N2 + k2) * eb], &a[((size_t)k1 + (size_t)N1 * k2) * eb], eb);
+    return hip_ok(hipMemcpy(out.p, b.data(), b.size(), hipMemcpyHostToDevice), "conv table rows");
+}
+
 bool Plan1D::build() {
     if (!runtime_init())
         return false;
@@ -938,7 +961,9 @@ bool Plan1D::build() {
             return false;
         if (!fft_table_f64(L, b64.p, bhat_, prec_, 1.0))
             return false;
-        return sub_.plan(L, prec_, var_.four, var_.maxr);
+        if (!sub_.plan(L, prec_, var_.four, var_.maxr))
+            return false;
+        return conv_table_rows(bhat_, sub_, hperm_);
     }
     kind_ = Kind::Bluestein;
     M_ = smooth_at_least(2ull * n_ - 1, prec_);
@@ -961,21 +986,7 @@ bool Plan1D::build() {
         return false;
     if (!sub_.plan(M_, prec_, var_.four, var_.maxr))
         return false;
-    if (sub_.four) {
-        // the convolution's middle pass reads h in the row order of its spectrum
-        const size_t eb = elem_bytes(prec_);
-        std::vector<unsigned char> a((size_t)M_ * eb), b((size_t)M_ * eb);
-        if (!hip_ok(hipMemcpy(a.data(), hhat_.p, a.size(), hipMemcpyDeviceToHost), "hhat") ||
-            !hperm_.alloc(b.size()))
-            return false;
-        const uint32_t N1 = sub_.N1, N2 = sub_.N2;
-        for (uint32_t k1 = 0; k1 < N1; ++k1)
-            for (uint32_t k2 = 0; k2 < N2; ++k2)
-                memcpy(&b[((size_t)k1 * N2 + k2) * eb], &a[((size_t)k1 + (size_t)N1 * k2) * eb], eb);
-        if (!hip_ok(hipMemcpy(hperm_.p, b.data(), b.size(), hipMemcpyHostToDevice), "hperm"))
-            return false;
-    }
-    return true;
+    return conv_table_rows(hhat_, sub_, hperm_);
 }
 
 void cost_features(uint32_t n, uint32_t kind, uint32_t count, int prec, const RaggedChoice& c,
@@ -1080,9 +1091,9 @@ bool Plan1D::supports_plain_columns() const {
 size_t Plan1D::scratch_bytes(uint64_t batch) const {
     const size_t eb = elem_bytes(prec_);
     size_t elems = sub_.scratch_elems(batch);
-    if (kind_ == Kind::Rader)
-        elems += batch + (size_t)batch * sub_.L;
-    else if (kind_ == Kind::Bluestein)
+    if (kind_ == Kind::Rader) // x0, A[0] (each 256-byte aligned), the convolution's scratch
+        return (2 * ((batch * eb + 255) / 256) * 256) + (size_t)batch * sub_.L * eb + 256;
+    if (kind_ == Kind::Bluestein)
         elems = (size_t)batch * sub_.L; // the convolution's four-step scratch
     return elems * eb + 256;
 }
@@ -1125,32 +1136,24 @@ hipError_t Plan1D::execute(const Layout& in, const Layout& out, uint64_t batch, 
     if (fuse && fuse->pw_in)
         return hipErrorInvalidValue; // callers apply the product separately for these kinds
     const uint32_t L = sub_.L;
-    const uint32_t dft = fuse ? fuse->dft : 0u;
     char* sp = (char*)scratch;
     if (kind_ == Kind::Rader) {
         const bool real_in = (in.flags & KFA_IO_REAL) != 0;
         const bool real_out = (out.flags & KFA_IO_REAL) != 0;
         void* x0 = sp;
         sp += ((batch * eb + 255) / 256) * 256;
-        void* tmpA = sp;
-        sp += (size_t)batch * L * eb;
-        void* sub_scr = sp;
+        void* dc = sp;
+        sp += ((batch * eb + 255) / 256) * 256;
+        void* tmp = sp;
         hipError_t e = launch_gather_first(prec_, real_in, in.ptr, in.dist_hi, in.dist_lo,
                                            in.C ? in.C : 1, x0, batch, s, in.inst);
         if (e != hipSuccess)
             return e;
-        // pass 1: a[n] = x'[g^n], A = FFT_{p-1}(a)
+        // one convolution: a[n] = x'[g^n], A = FFT_{p-1}(a) (A[0] saved to dc),
+        // c = IFFT(A * FFT(b)); X[g^-m] = x0 + c[m]
         KfaIo l1 = load;
         l1.flags |= KFA_IO_MAP | (inverse ? KFA_IO_CONJ1 : 0);
         l1.map = (uint64_t)qidx_.p;
-        KfaIo s1 = io_plain(tmpA, L);
-        e = sub_.run(l1, s1, 1.0, 1.0, batch, sub_scr, s, false, dft);
-        if (e != hipSuccess)
-            return e;
-        // pass 2: c = IFFT(A * FFT(b)); X[g^-m] = x0 + c[m]
-        KfaIo l2 = io_plain(tmpA, L);
-        l2.flags |= KFA_IO_PW | KFA_IO_CONJ2;
-        l2.pw = (uint64_t)bhat_.p;
         KfaIo s2 = store;
         s2.flags |= KFA_IO_MAP | KFA_IO_CONJ1 | KFA_IO_SCALE | KFA_IO_ADDX0 |
                     (inverse ? KFA_IO_CONJ2 : 0) | (scaled ? KFA_IO_SCALE_POST : 0) |
@@ -1159,11 +1162,11 @@ hipError_t Plan1D::execute(const Layout& in, const Layout& out, uint64_t batch, 
         s2.aux = (uint64_t)x0;
         s2.aux_dist = (s2.C > 1) ? s2.C : 1; // x0 is indexed by instance = hi*C + lo
         s2.aux_dist_lo = (s2.C > 1) ? 1 : 0;
-        e = sub_.run(l2, s2, 1.0 / (double)L, scale, batch, sub_scr, s, false, dft);
+        e = sub_.run_conv(l1, s2, 1.0 / (double)L, scale, batch, bhat_.p, hperm_.p, tmp, s, dc);
         if (e != hipSuccess)
             return e;
         // X[0] = x0 + A[0]
-        return launch_rader_dc(prec_, x0, real_in, tmpA, L, out.ptr, out.dist_hi, out.dist_lo,
+        return launch_rader_dc(prec_, x0, real_in, dc, 1, out.ptr, out.dist_hi, out.dist_lo,
                                out.C ? out.C : 1, real_out, inverse, scale, batch, s, out.inst);
     }
     // Bluestein: X = chirp . IDFT_M(DFT_M(chirp . x, zero padded) * FFT(h)) / M
@@ -1176,7 +1179,7 @@ hipError_t Plan1D::execute(const Layout& in, const Layout& out, uint64_t batch, 
     s2.flags |= KFA_IO_CONJ1 | KFA_IO_PW | KFA_IO_SCALE | (inverse ? KFA_IO_CONJ2 : 0);
     s2.pw = (uint64_t)chirp_.p;
     s2.valid = n_;
-    return sub_.run_conv(l1, s2, scale / (double)L, batch, hhat_.p, hperm_.p, tmp, s);
+    return sub_.run_conv(l1, s2, scale / (double)L, 1.0, batch, hhat_.p, hperm_.p, tmp, s);
 }
 
 // =============================================================== plan registry

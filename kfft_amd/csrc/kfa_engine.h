@@ -121,8 +121,9 @@ struct SubFft {
     // operations (the store's CONJ1 completes the inverse): one pass, or three
     // for four-step lengths, where hperm holds h[k1 + N1 k2] at [k1][k2] and
     // scratch holds batch x L values
-    hipError_t run_conv(const KfaIo& load, const KfaIo& store, double scale, uint64_t batch,
-                        const void* h, const void* hperm, void* scratch, hipStream_t s) const;
+    hipError_t run_conv(const KfaIo& load, const KfaIo& store, double scale, double scale_post,
+                        uint64_t batch, const void* h, const void* hperm, void* scratch,
+                        hipStream_t s, void* dc = nullptr) const;
     // both passes on columns of row-major matrices (C = columns, dist_lo = 1): the
     // shaped column passes walk the columns with the lanes
     hipError_t run_four_cols(const KfaIo& load, const KfaIo& store, double scale, uint64_t batch,
@@ -198,7 +199,8 @@ class Plan1D {
     uint32_t M_ = 0;
     DevBuf qidx_, pidx_, bhat_; // Rader
     DevBuf chirp_, hhat_;       // Bluestein
-    DevBuf hperm_;              // Bluestein, four-step M: hhat[k1 + N1 k2] at [k1][k2]
+    DevBuf hperm_;              // four-step inner length: the FFT(h) / FFT(b) table at
+                                // [k1][k2] holding index k1 + N1 k2 (convolution pass)
     mutable std::string desc_;
 };
 

@@ -165,6 +165,8 @@ __device__ __forceinline__ void fft_tile(const KfaFftArgs& A, uint64_t first, cx
     if (round == 1) {
         if (act) {
             const uint64_t row = (first + il) % A.mid_rows;
+            if (A.mid_dc && row == 0 && lane == 0)
+                ((cx<T>*)A.mid_dc)[(first + il) / A.mid_rows] = my[lds_index<T>(0)];
             const cx<T>* h = (const cx<T>*)A.mid_pw + row * L;
             for (uint32_t e = lane; e < L; e += tpi) {
                 cx<T> v = cmul(my[lds_index<T>(e)], h[e]);
