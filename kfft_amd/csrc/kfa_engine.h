@@ -259,8 +259,9 @@ struct RaggedOptions {
 // pass.  All scaled to about [0, 1].
 constexpr int COST_FEATURES = 19;
 // ln of the predicted speed-up a choice needs over the default shape before the
-// tuned planner takes it (5%)
-constexpr float COST_MARGIN = 0.0488f;
+// tuned planner takes it (25%: the default shapes are close to the optimum of the
+// choices, and smaller margins let prediction errors pick slower shapes)
+constexpr float COST_MARGIN = 0.2231f;
 void cost_features(uint32_t n, uint32_t kind, uint32_t count, int prec, const RaggedChoice& c,
                    float* out);
 // the choices the autotuner searches for one class, and whether one applies to
