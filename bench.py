@@ -245,7 +245,7 @@ def _timed_async(plan, x, y, batch, steps, half):
     return e0.elapsed_time(e1) / steps
 
 
-def other_configs(steps: int):
+def other_configs(steps: int, only: str = None):
     """The reference's other workloads at their full sizes, plain FP64 unless
     stated: time, algorithmic GB/s against the 8 TB/s roofline, and the
     relative L2 error of a sample against the oracle (or the stated check)."""
@@ -256,6 +256,9 @@ def other_configs(steps: int):
     import oracle as O
 
     out = {}
+
+    def want(name):  # all workloads, or only the named one (profiler runs)
+        return only is None or name == only
 
     def entry(name, ms, alg_bytes, err, **kw):
         gbs = alg_bytes / (ms * 1e-3) / 1e9
@@ -274,19 +277,22 @@ def other_configs(steps: int):
         return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
     # 4096 x 256 FP64: forward, then inverse with DISABLE_NORM (checked as inverse / N)
-    n, b = 4096, 256
-    x = cplx(b, n)
-    y, z = torch.empty_like(x), torch.empty_like(x)
-    fw, iv = K.CpxPlan(n), K.CpxPlan(n, K.FLAG_INVERSE | K.FLAG_DISABLE_NORM)
-    tf = _timed_async(fw, x, y, b, steps * 10, False)
-    ti = _timed_async(iv, y, z, b, steps * 10, False)
-    err = max(rel(y[:2].cpu().numpy(), O.fft(x[:2].cpu().numpy())),
-              rel((z / n).cpu().numpy(), x.cpu().numpy()))
-    entry("c2c_4096x256_fwd_inv_f64", tf + ti, 2 * 2.0 * n * 16 * b, err,
-          fwd_ms=round(tf, 4), inv_ms=round(ti, 4))
-    del x, y, z
+    if want("c2c_4096x256_fwd_inv_f64"):
+        n, b = 4096, 256
+        x = cplx(b, n)
+        y, z = torch.empty_like(x), torch.empty_like(x)
+        fw, iv = K.CpxPlan(n), K.CpxPlan(n, K.FLAG_INVERSE | K.FLAG_DISABLE_NORM)
+        tf = _timed_async(fw, x, y, b, steps * 10, False)
+        ti = _timed_async(iv, y, z, b, steps * 10, False)
+        err = max(rel(y[:2].cpu().numpy(), O.fft(x[:2].cpu().numpy())),
+                  rel((z / n).cpu().numpy(), x.cpu().numpy()))
+        entry("c2c_4096x256_fwd_inv_f64", tf + ti, 2 * 2.0 * n * 16 * b, err,
+              fwd_ms=round(tf, 4), inv_ms=round(ti, 4))
+        del x, y, z
     # hard lengths, ~256 MB per set
     for n, b in [(1000, 16384), (6561, 2560), (1009, 16384), (65537, 256), (1 << 20, 16)]:
+        if not want(f"c2c_{n}x{b}_f64"):
+            continue
         x = cplx(b, n)
         y = torch.empty_like(x)
         p = K.CpxPlan(n)
@@ -295,33 +301,37 @@ def other_configs(steps: int):
         entry(f"c2c_{n}x{b}_f64", ms, 2.0 * n * 16 * b, err, plan=p.describe())
         del x, y
     # real input 2^18 x 512
-    n, b = 1 << 18, 512
-    g = torch.Generator(device="cuda").manual_seed(2)
-    xr = torch.rand(b, n, generator=g, device="cuda", dtype=torch.float64) * 2 - 1
-    sp = K.ScalarPlan(n)
-    yr = torch.empty(b, n // 2 + 1, dtype=torch.complex128, device="cuda")
-    L = K.load(False)
-    ms = _timed(lambda: L.kfft_eval_scalar_batch(sp.handle, xr.data_ptr(), yr.data_ptr(), b),
-                steps)
-    err = rel(yr[0].cpu().numpy(), O.rfft(xr[0].cpu().numpy()))
-    entry("r2c_262144x512_f64", ms, b * (n * 8.0 + (n // 2 + 1) * 16.0), err)
-    del xr, yr
+    if want("r2c_262144x512_f64"):
+        n, b = 1 << 18, 512
+        g = torch.Generator(device="cuda").manual_seed(2)
+        xr = torch.rand(b, n, generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+        sp = K.ScalarPlan(n)
+        yr = torch.empty(b, n // 2 + 1, dtype=torch.complex128, device="cuda")
+        L = K.load(False)
+        ms = _timed(lambda: L.kfft_eval_scalar_batch(sp.handle, xr.data_ptr(), yr.data_ptr(), b),
+                    steps)
+        err = rel(yr[0].cpu().numpy(), O.rfft(xr[0].cpu().numpy()))
+        entry("r2c_262144x512_f64", ms, b * (n * 8.0 + (n // 2 + 1) * 16.0), err)
+        del xr, yr
     # 2D 8192 x 8192 complex FP32, forward and inverse
-    n = 8192
-    a = cplx(n, n, torch.complex64, seed=3)
-    f2, i2 = K.Cpx2Plan(n, n, half=True), K.Cpx2Plan(n, n, K.FLAG_INVERSE, half=True)
-    A = torch.empty_like(a)
-    B = torch.empty_like(a)
-    L32 = K.load(True)
-    tf = _timed(lambda: L32.kfft_eval2_cpx(f2.handle, a.data_ptr(), A.data_ptr()), steps)
-    ti = _timed(lambda: L32.kfft_eval2_cpx(i2.handle, A.data_ptr(), B.data_ptr()), steps)
-    err = rel(B.to(torch.complex128).cpu().numpy(), a.to(torch.complex128).cpu().numpy())
-    entry("c2c_2d_8192x8192_f32", tf + ti, 2 * 2.0 * n * n * 8, err, fwd_ms=round(tf, 4),
-          inv_ms=round(ti, 4), check="inverse(forward(x)) against x")
-    del a, A, B
+    if want("c2c_2d_8192x8192_f32"):
+        n = 8192
+        a = cplx(n, n, torch.complex64, seed=3)
+        f2, i2 = K.Cpx2Plan(n, n, half=True), K.Cpx2Plan(n, n, K.FLAG_INVERSE, half=True)
+        A = torch.empty_like(a)
+        B = torch.empty_like(a)
+        L32 = K.load(True)
+        tf = _timed(lambda: L32.kfft_eval2_cpx(f2.handle, a.data_ptr(), A.data_ptr()), steps)
+        ti = _timed(lambda: L32.kfft_eval2_cpx(i2.handle, A.data_ptr(), B.data_ptr()), steps)
+        err = rel(B.to(torch.complex128).cpu().numpy(), a.to(torch.complex128).cpu().numpy())
+        entry("c2c_2d_8192x8192_f32", tf + ti, 2 * 2.0 * n * n * 8, err, fwd_ms=round(tf, 4),
+              inv_ms=round(ti, 4), check="inverse(forward(x)) against x")
+        del a, A, B
     # real 2D convolution: 4096^2 image (uniform[0,1)) with a 63^2 Gaussian plus noise kernel,
     # both zero-padded to 4160^2; the reference's even-y quirk makes the output 2x the
     # circular convolution.  Checked against an FP64 direct sum on a 64 x 64 crop.
+    if not (want("conv2d_real_4160x4160_f32") or want("conv2d_real_4160x4160_f64")):
+        return out
     N, M, k = 4160, 4096, 63
     rng = np.random.default_rng(4)
     img = np.zeros((N, N))
@@ -337,6 +347,8 @@ def other_configs(steps: int):
         for dx in range(k):
             crop += ker[dy, dx] * win[k - 1 - dy:k - 1 - dy + 64, k - 1 - dx:k - 1 - dx + 64]
     for half in (True, False):
+        if not want(f"conv2d_real_4160x4160_{'f32' if half else 'f64'}"):
+            continue
         dt = torch.float32 if half else torch.float64
         ia = torch.from_numpy(img).to(dt).cuda()
         ka = torch.from_numpy(kp).to(dt).cuda()
@@ -405,6 +417,10 @@ def main():
                          "2^18, 2D 8192^2, real conv2d 4160^2)")
     ap.add_argument("--onchip", action="store_true",
                     help="plan with KFFT_FLAG_GPU_ONCHIP (fused persistent two-pass kernel)")
+    ap.add_argument("--config", default=None,
+                    help="time one of the reference's other workloads alone (a key of the "
+                         "'configs' object, e.g. c2c_1000x16384_f64) and print its entry: for "
+                         "profiler runs")
     ap.add_argument("--copy-ref", action="store_true",
                     help="after timing, run one plain device copy of the FP64 input (4 GiB at the "
                          "default size) so that profiler byte counters can be calibrated")
@@ -418,6 +434,13 @@ def main():
     import numpy as np
     import torch
 
+    if args.config:
+        res = other_configs(max(1, args.steps), only=args.config)
+        if args.config not in res:
+            print(json.dumps({"error": f"unknown workload {args.config}"}))
+            sys.exit(2)
+        print(json.dumps({"config": args.config, **res[args.config]}))
+        return
     n, batch = args.n, args.batch
     forms = [f for f in FORMS if f in args.forms.split(",") or f in (args.form, "fp64")]
     if args.only:
