@@ -296,7 +296,7 @@ bool SubFft::plan(uint32_t len, int p, bool force_four, uint32_t max_radix) {
     if (!hip_ok(launch_twiddle_table(tw4_lo.p, S, 1, L, s), "tw4 lo") ||
         !hip_ok(launch_twiddle_table(tw4_hi.p, nhi, S, L, s), "tw4 hi"))
         return false;
-    if (fast_len(N1) && fast_len(N2) && maxr >= 16) { // shaped passes use the default radices
+    if (fast_len_split(N1) && fast_len_split(N2) && maxr >= 16) { // shaped passes use the default radices
         DevBuf full64;
         if (!full64.alloc((size_t)L * 16) || !tw4_full.alloc((size_t)L * elem_bytes(prec)))
             return false;
@@ -700,7 +700,7 @@ hipError_t SubFft::run(const KfaIo& load, const KfaIo& store, double scale, doub
         if (simple && fast_len(L) && !rec && maxr >= 16) {
             const bool rows = load.C == 1 && store.C == 1 && load.stride == 1 && store.stride == 1;
             const bool cols = load.C == store.C && load.dist_lo == 1 && store.dist_lo == 1 &&
-                              load.stride != 1 && store.stride != 1;
+                              load.stride != 1 && store.stride != 1 && fast_len_split(L);
             if (rows || cols) {
                 FastArgs f = fast_args(direct.tw.p, batch, scale);
                 f.in = (const void*)load.ptr;

@@ -1,4 +1,5 @@
-// Compile-time shaped FFT passes for power-of-two lengths 16..4096.
+// Compile-time shaped FFT passes for power-of-two lengths 16..4096 (row passes
+// also 8192).
 //
 // These carry the batched C2C hot path (one-pass transforms and both passes of
 // a four-step transform).  Every thread owns E = 16 complex values (8 for
@@ -32,7 +33,7 @@ enum { FAST_ROW = 0, FAST_COL = 1, FAST_TOUT = 2 };
 // a = 4q+1, q > 0)
 template <int L>
 struct P2 {
-    static constexpr int a = (L >= 4096) ? 12 : (L >= 2048) ? 11 : (L >= 1024) ? 10 : (L >= 512) ? 9
+    static constexpr int a = (L >= 8192) ? 13 : (L >= 4096) ? 12 : (L >= 2048) ? 11 : (L >= 1024) ? 10 : (L >= 512) ? 9
                            : (L >= 256) ? 8 : (L >= 128) ? 7 : (L >= 64) ? 6 : (L >= 32) ? 5 : 4;
     static constexpr int q = a / 4, rr = a % 4;
     static constexpr int S = (rr == 0) ? q : (rr == 1 && q > 0) ? q + 1 : q + 1;
@@ -380,6 +381,10 @@ static hipError_t launch_fast_mode(uint32_t L, const FastArgs& a, hipStream_t s)
     case 1024: return launch_fast_sel<T, 1024, MODE, DK>(a, s);
     case 2048: return launch_fast_sel<T, 2048, MODE, DK>(a, s);
     case 4096: return launch_fast_sel<T, 4096, MODE, DK>(a, s);
+    case 8192:
+        if constexpr (MODE == FAST_ROW) // one instance per workgroup: rows only
+            return launch_fast_sel<T, 8192, MODE, DK>(a, s);
+        return hipErrorNotSupported;
     default: return hipErrorNotSupported;
     }
 }

@@ -47,7 +47,11 @@ hipError_t launch_ragged(int prec, const RagArgs& r, uint32_t groups, uint32_t l
 hipError_t launch_ragged_patch(void* blob, const uint64_t* off, const uint32_t* which,
                                const int64_t* delta, uint32_t n, const void* in, void* out,
                                hipStream_t s);
-inline bool fast_len(uint32_t L) { return L >= 16 && L <= 4096 && (L & (L - 1)) == 0; }
+// lengths of the shaped row passes (one instance of 8192 fills a 512-thread
+// workgroup) and of the shaped column / transposing passes of the two-pass
+// transforms (which need several instances per workgroup to coalesce)
+inline bool fast_len(uint32_t L) { return L >= 16 && L <= 8192 && (L & (L - 1)) == 0; }
+inline bool fast_len_split(uint32_t L) { return L >= 16 && L <= 4096 && (L & (L - 1)) == 0; }
 hipError_t launch_stage_twiddles(int prec, void* tw, const KfaFftArgs& a, hipStream_t s);
 hipError_t launch_twiddle_table(void* out_d, uint64_t count, uint64_t step, uint64_t M,
                                 hipStream_t s);

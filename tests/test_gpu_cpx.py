@@ -207,3 +207,18 @@ def test_fused_two_pass_path(half, batch):
     err = (torch.linalg.vector_norm((z - xin).to(torch.complex128)) /
            torch.linalg.vector_norm(xin.to(torch.complex128))).item()
     assert err <= bar
+
+
+@pytest.mark.parametrize("n", [4096, 8192, 16384])
+def test_float_build_power_of_two_rows(n):
+    """FP32 build: one-pass shaped rows up to 8192 (one instance per 512-thread
+    workgroup at 8192) and the two-pass length after it, both directions, in a
+    batch that leaves the last workgroups partly idle."""
+    rng = np.random.default_rng(n + 3)
+    x = cplx(rng, 5, n).astype(np.complex64)
+    X = K.CpxPlan(n, half=True)(x)
+    ref = O.fft(x.astype(np.complex128))
+    assert rel(X, ref) <= 5e-7 * np.log2(n)
+    inv = K.CpxPlan(n, K.FLAG_INVERSE, half=True)(ref.astype(np.complex64))
+    assert rel(inv, O.fft(ref.astype(np.complex64).astype(np.complex128), O.FLAG_INVERSE)) <= \
+        5e-7 * np.log2(n)

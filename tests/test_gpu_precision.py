@@ -42,8 +42,8 @@ def tones32(rng, batch, n, amp=1e-6, count=5):
     return x.astype(np.complex64)
 
 
-# lengths with radix-16 stages: one-pass 16..4096, two-pass 2^16 and 2^20
-SIZES = [16, 64, 256, 512, 1024, 2048, 4096, 1 << 16, 1 << 20]
+# lengths with radix-16 stages: one-pass 16..8192, two-pass 2^16 and 2^20
+SIZES = [16, 64, 256, 512, 1024, 2048, 4096, 8192, 1 << 16, 1 << 20]
 
 
 @pytest.mark.parametrize("form", ["f16", "bf16"])
