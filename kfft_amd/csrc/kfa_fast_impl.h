@@ -359,7 +359,7 @@ static inline int fast_cols_mul() {
 
 template <typename T, int L, int MODE, int DK>
 static hipError_t launch_fast_sel(const FastArgs& a, hipStream_t s) {
-    if constexpr (MODE != FAST_ROW && L >= 128 && L <= 1024) {
+    if constexpr (MODE != FAST_ROW && L >= 64 && L <= 1024) {
         const int m = fast_cols_mul();
         if (m == 2)
             return launch_fast_t<T, L, MODE, 2, DK>(a, s);
