@@ -162,41 +162,41 @@ __device__ __forceinline__ void fft_tile(const KfaFftArgs& A, uint64_t first, cx
     cx<T>* my = lds + (size_t)(act ? il : 0) * A.inst_stride;
     const uint32_t rounds = A.mid_pw ? 2u : 1u;
     for (uint32_t round = 0; round < rounds; ++round) {
-    if (round == 1) {
-        if (act) {
-            const uint64_t row = (first + il) % A.mid_rows;
-            if (A.mid_dc && row == 0 && lane == 0)
-                ((cx<T>*)A.mid_dc)[(first + il) / A.mid_rows] = my[lds_index<T>(0)];
-            const cx<T>* h = (const cx<T>*)A.mid_pw + row * L;
-            for (uint32_t e = lane; e < L; e += tpi) {
-                cx<T> v = cmul(my[lds_index<T>(e)], h[e]);
-                v.im = -v.im;
-                my[lds_index<T>(e)] = v;
+        if (round == 1) {
+            if (act) {
+                const uint64_t row = (first + il) % A.mid_rows;
+                if (A.mid_dc && row == 0 && lane == 0)
+                    ((cx<T>*)A.mid_dc)[(first + il) / A.mid_rows] = my[lds_index<T>(0)];
+                const cx<T>* h = (const cx<T>*)A.mid_pw + row * L;
+                for (uint32_t e = lane; e < L; e += tpi) {
+                    cx<T> v = cmul(my[lds_index<T>(e)], h[e]);
+                    v.im = -v.im;
+                    my[lds_index<T>(e)] = v;
+                }
+            }
+            __syncthreads();
+        }
+        for (uint32_t s = 0; s < A.nstages; ++s) {
+            switch (A.st[s].radix) {
+                KFA_STAGE_CASE(2)
+                KFA_STAGE_CASE(3)
+                KFA_STAGE_CASE(4)
+                KFA_STAGE_CASE(5)
+                KFA_STAGE_CASE(7)
+                KFA_STAGE_CASE(8)
+                KFA_STAGE_CASE(9)
+                KFA_STAGE_CASE(11)
+                KFA_STAGE_CASE(13)
+                KFA_STAGE_CASE(16)
+                KFA_STAGE_CASE(17)
+                KFA_STAGE_CASE(19)
+                KFA_STAGE_CASE(23)
+                KFA_STAGE_CASE(29)
+                KFA_STAGE_CASE(31)
+            default:
+                break;
             }
         }
-        __syncthreads();
-    }
-    for (uint32_t s = 0; s < A.nstages; ++s) {
-        switch (A.st[s].radix) {
-            KFA_STAGE_CASE(2)
-            KFA_STAGE_CASE(3)
-            KFA_STAGE_CASE(4)
-            KFA_STAGE_CASE(5)
-            KFA_STAGE_CASE(7)
-            KFA_STAGE_CASE(8)
-            KFA_STAGE_CASE(9)
-            KFA_STAGE_CASE(11)
-            KFA_STAGE_CASE(13)
-            KFA_STAGE_CASE(16)
-            KFA_STAGE_CASE(17)
-            KFA_STAGE_CASE(19)
-            KFA_STAGE_CASE(23)
-            KFA_STAGE_CASE(29)
-            KFA_STAGE_CASE(31)
-        default:
-            break;
-        }
-    }
     }
 
     // 3. store, thread mapping as the load's (out_jfast)
