@@ -304,8 +304,14 @@ constexpr int fast_ipw() {
     constexpr int PSH = sizeof(T) == 8 ? KFA_PAD_SHIFT_F64 : KFA_PAD_SHIFT_F32;
     constexpr int bytes = (int)sizeof(cx<T>) * ((P2<L>::PAD_L(PSH)) | 1); // per instance
     constexpr int thr = (1024 / TPI) > 0 ? (1024 / TPI) : 1;
-    constexpr int cap = (MUL == 1 ? 65536 : 160 * 1024) / bytes > 0
-                            ? (MUL == 1 ? 65536 : 160 * 1024) / bytes : 1;
+    // widened column tiles (MUL > 1) may fill the CU's LDS, except at 1024
+    // points, where two workgroups of at most 80 KiB per CU (a power of two of
+    // instances) are faster (measured: the 1024 x 1024 two-pass transform 23%
+    // faster; 512-point transposing passes are faster with one full workgroup)
+    constexpr int lim = MUL == 1 ? 65536 : (L >= 1024 ? 80 * 1024 : 160 * 1024);
+    constexpr int fit = lim / bytes > 0 ? lim / bytes : 1;
+    constexpr int cap = (MUL == 1 || L < 1024) ? fit : (fit >= 16 ? 16 : fit >= 8 ? 8
+                                                        : fit >= 4 ? 4 : fit >= 2 ? 2 : 1);
     if (MODE == FAST_ROW) {
         constexpr int want = 256 / TPI > 0 ? 256 / TPI : 1;
         constexpr int c1 = (65536 / bytes) > 0 ? (65536 / bytes) : 1;
