@@ -478,8 +478,9 @@ static kfft_return_t sclr_fwd_dev(kfft_plan_sclr* P, const void* din, void* dout
         return hip_ok(e, "scalar fwd") ? KFFT_RET_SUCCESS : KFFT_RET_GPU_FAIL;
     }
     const uint32_t h = n / 2;
-    if (!expand && pl.fuses_real_post() && !pl.shaped_rows()) {
-        // one LDS pass: Z = FFT_h(x viewed as h complex) rebuilt into X in its store
+    if (!expand && ((pl.fuses_real_post() && !pl.shaped_rows()) || pl.fuses_real_post_shaped())) {
+        // one LDS pass, or the shaped two-pass form: Z = FFT_h(x viewed as h complex)
+        // rebuilt into X in the last pass's store
         Layout o = contiguous(dout, out_dist);
         o.real_post = g->st.p;
         hipError_t e = pl.execute(contiguous(din, h), o, batch, false, 1.0, g->scratch.p, s);

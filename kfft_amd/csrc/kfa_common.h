@@ -99,6 +99,10 @@ struct KfaFftArgs {
 #define KFA_FAST_TW4_HI 256u    // four-step twiddle indexed by (instance / out_C) mod pad
 #define KFA_FAST_MFMA_F16 64u   // radix-16 stages as split FP16 matrix products (FP32 data)
 #define KFA_FAST_MFMA_BF16 128u // radix-16 stages as split BF16 matrix products (FP32 data)
+// pass B of a real-input transform's half-length two-pass transform (TOUT): the
+// workgroup holds rows a and N1 - a (row 0 with N1/2) and stores the real-input
+// spectrum X[0..h] (h = N1 * L) instead of Z; rpost_st: super twiddles
+#define KFA_FAST_RPOST 512u
 
 struct FastArgs {
     const void* in;
@@ -112,6 +116,7 @@ struct FastArgs {
     uint32_t flags;
     uint32_t pad; // KFA_FAST_TW4_HI: the four-step row length N2 (twiddle table stride)
     double scale;
+    const void* rpost_st; // KFA_FAST_RPOST: st[i] = exp(-i pi ((i+1)/h + 1/2)), i < h
 };
 
 

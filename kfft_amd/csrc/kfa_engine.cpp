@@ -465,6 +465,12 @@ hipError_t SubFft::run_four_fast(const KfaIo& load, const KfaIo& store, double s
     b.flags = ((store.flags & KFA_IO_CONJ2) ? KFA_FAST_CONJ_OUT : 0) |
               ((store.flags & KFA_IO_SCALE) ? KFA_FAST_SCALE : 0) |
               ((nt & 2) ? KFA_FAST_NT_OUT : 0) | ((nt & 8) ? KFA_FAST_NT_IN : 0) | dft;
+    if (store.flags & KFA_IO_RPOST) {
+        // real-input rebuild in the store: X[0..L] of the 2L reals (no conjugation
+        // or scaling reaches here)
+        b.flags |= KFA_FAST_RPOST;
+        b.rpost_st = (const void*)store.pw;
+    }
     return launch_fast(prec, N2, 2, b, s);
 }
 
@@ -690,12 +696,28 @@ int SubFft::fused_error() const {
     return (int)v;
 }
 
+bool SubFft::rpost_two_pass() const {
+    if (!four || !tw4_full.p || N2 < 32 || N1 < 2)
+        return false;
+    const uint32_t ipw = prec == 8 ? fast_tout_ipw_f64(N2) : fast_tout_ipw_f32(N2);
+    return ipw >= 2 && ipw % 2 == 0 && N1 % ipw == 0;
+}
+
 hipError_t SubFft::run(const KfaIo& load, const KfaIo& store, double scale, double scale_post,
                        uint64_t batch, void* scratch, hipStream_t s, bool onchip,
                        uint32_t dft) const {
     const bool simple = fast_io_ok(load, KFA_IO_CONJ1) &&
                         fast_io_ok(store, KFA_IO_CONJ2 | KFA_IO_SCALE) && scale_post == 1.0;
     const bool rec = recorder() != nullptr; // recording for the ragged executor: general passes
+    if (four && (store.flags & KFA_IO_RPOST)) {
+        // fused rebuild: the shaped two-pass form on plain contiguous rows only
+        KfaIo st = store;
+        st.flags &= ~KFA_IO_RPOST;
+        if (rec || !rpost_two_pass() || load.C != 1 || store.C != 1 || scale_post != 1.0 ||
+            !fast_io_ok(load, 0) || !fast_io_ok(st, 0))
+            return hipErrorInvalidValue;
+        return run_four_fast(load, store, scale, batch, scratch, tuning().nt, s, dft);
+    }
     if (!four) {
         if (simple && fast_len(L) && !rec && maxr >= 16) {
             const bool rows = load.C == 1 && store.C == 1 && load.stride == 1 && store.stride == 1;
@@ -1115,7 +1137,7 @@ hipError_t Plan1D::execute(const Layout& in, const Layout& out, uint64_t batch, 
     const size_t eb = elem_bytes(prec_);
     KfaIo load = io_from(in), store = io_from(out);
     const bool scaled = scale != 1.0;
-    if (out.real_post && (!fuses_real_post() || inverse || scaled))
+    if (out.real_post && (!(fuses_real_post() || fuses_real_post_shaped()) || inverse || scaled))
         return hipErrorInvalidValue;
     if (kind_ == Kind::Identity || kind_ == Kind::Direct) {
         if (fuse && fuse->pw_in) {

@@ -133,6 +133,9 @@ struct SubFft {
                          hipStream_t s) const;
     // non-zero if a fused launch since the last check timed out waiting (syncs the stream)
     int fused_error() const;
+    // shaped two-pass form whose transposing pass can hold rows k1 and N1 - k1 in
+    // one workgroup (the real-input rebuild fused into its store, KFA_IO_RPOST)
+    bool rpost_two_pass() const;
     mutable std::mutex fused_mu;
     mutable DevBuf fused_ctr, fused_ring;
     mutable uint64_t fused_T = 0;
@@ -188,6 +191,9 @@ class Plan1D {
     // outside the ragged executor, plain rows of this length run a shaped pass
     // (kfa_fast_impl.h), which has no fused rebuild
     bool shaped_rows() const;
+    // shaped two-pass form with the rebuild fused into its second pass (plain
+    // contiguous rows, outside the ragged executor)
+    bool fuses_real_post_shaped() const { return kind_ == Kind::Direct && sub_.rpost_two_pass(); }
     int async_error() const { return sub_.fused_error(); }
 
   private:

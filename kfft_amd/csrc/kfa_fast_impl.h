@@ -136,10 +136,24 @@ __device__ __forceinline__ void fast_tile(const FastArgs& A, uint64_t first, cx<
     // split FP16: whether stage 0's inputs were narrow enough that stage 1 needs
     // no range test (decided for the whole workgroup at the first barrier)
     bool narrow0 = true, skip_check1 = false;
+    // real-input rebuild (TOUT): slot j < IPW/2 of tile t of a transform holds
+    // row a = t IPW/2 + j, slot j + IPW/2 its partner N1 - a (row N1/2 for a = 0),
+    // so that every bin's partner bin h - k is in the workgroup (IPW divides N1)
+    const bool rpost = MODE == FAST_TOUT && (A.flags & KFA_FAST_RPOST) != 0;
+    auto slot_inst = [&](uint32_t j) -> uint64_t {
+        if (!rpost)
+            return first + j;
+        const uint32_t n1 = A.out_C, half = IPW / 2;
+        const uint64_t b = first / n1;
+        const uint32_t t = (uint32_t)(first - b * n1) / IPW;
+        const uint32_t a = t * half + (j % half);
+        const uint32_t row = j < half ? a : (a == 0 ? n1 / 2 : n1 - a);
+        return b * n1 + row;
+    };
     // ---- stage 0: HBM -> registers, DFT_R0, registers -> LDS (or HBM)
     {
         constexpr int R = S::R(0), NB = E / R, STR = L / R;
-        const uint64_t inst = first + li;
+        const uint64_t inst = slot_inst(li);
         // threads past the batch read the tile's first instance (always valid) and
         // store nothing, so the loads need no per-element branches
         const uint64_t ld = inst < A.batch ? inst : first;
@@ -234,6 +248,55 @@ __device__ __forceinline__ void fast_tile(const FastArgs& A, uint64_t first, cx<
                     my[e + (e >> PSH)] = v[i * R + r];
                 }
             }
+        }
+    }
+    // ---- real-input rebuild: the rows' Z through LDS, then X[k] from Z[k] and
+    // Z[h - k] (kfft_scalar.c:174-216), X[h] from Z[0]; h = N1 * L
+    if constexpr (MODE == FAST_TOUT && NST > 1) {
+        if (rpost) {
+            constexpr int RL = S::R(NST - 1), NB = E / RL, STR = L / RL;
+            __syncthreads(); // every read of the last stage's operands is done
+            cx<T>* my = lds + (size_t)si * SP;
+#pragma unroll
+            for (int i = 0; i < NB; ++i)
+#pragma unroll
+                for (int r = 0; r < RL; ++r) {
+                    const int e = (int)sl + i * TPI + r * STR;
+                    my[e + (e >> PSH)] = v[i * RL + r];
+                }
+            __syncthreads();
+            const uint32_t n1 = A.out_C, half = IPW / 2;
+            const uint64_t inst = slot_inst(si);
+            if (inst < A.batch) {
+                const uint64_t hi = inst / n1;
+                const uint32_t row = (uint32_t)(inst - hi * n1);
+                const uint32_t ps = (row == 0 || row == n1 / 2) ? si : (si < half ? si + half : si - half);
+                const cx<T>* mp = lds + (size_t)ps * SP;
+                const cx<T>* st = (const cx<T>*)A.rpost_st;
+                cx<T>* q0 = (cx<T>*)A.out + ((int64_t)hi * A.out_hi + (int64_t)row * A.out_lo);
+                // plain stores: the rows' halves run in opposite directions, and
+                // streaming stores of such runs measured 9% slower
+                auto put = [&](int64_t at, cx<T> x) { q0[at] = x; };
+#pragma unroll
+                for (int i = 0; i < NB; ++i)
+#pragma unroll
+                    for (int r = 0; r < RL; ++r) {
+                        const int e = (int)sl + i * TPI + r * STR;
+                        const cx<T> za = v[i * RL + r];
+                        if (row == 0 && e == 0) {
+                            put(0, {za.re + za.im, T(0)});
+                            put((int64_t)L * A.out_stride, {za.re - za.im, T(0)});
+                            continue;
+                        }
+                        const int pe = row == 0 ? L - e : L - 1 - e;
+                        const cx<T> zb = mp[pe + (pe >> PSH)];
+                        const uint32_t k = row + n1 * (uint32_t)e;
+                        cx<T> xk, xh;
+                        rebuild_pair<T>(za, zb, st[k - 1], xk, xh);
+                        put((int64_t)e * A.out_stride, xk);
+                    }
+            }
+            return;
         }
     }
     // ---- store: last stage outputs e = b + r * (L / R_last) (b < L / R_last)
@@ -331,6 +394,13 @@ static hipError_t launch_fast_t(const FastArgs& a, hipStream_t s) {
     const uint32_t lds = P2<L>::S > 1 ? (uint32_t)(IPW * SP * sizeof(cx<T>)) : 0u;
     static_assert(IPW * SP * sizeof(cx<T>) <= 160 * 1024, "LDS budget");
     static_assert(IPW * P2<L>::TPI <= 1024, "workgroup size");
+    // the rebuild pairs rows a and N1 - a inside one tile
+    // (whole transforms, no other store operations)
+    if ((a.flags & KFA_FAST_RPOST) &&
+        (MODE != FAST_TOUT || P2<L>::S < 2 || IPW < 2 || IPW % 2 || a.out_C % IPW ||
+         a.batch % a.out_C ||
+         (a.flags & (KFA_FAST_CONJ_OUT | KFA_FAST_SCALE | KFA_FAST_TW4 | KFA_FAST_TW4_HI))))
+        return hipErrorNotSupported;
     static bool attr = false;
     if (!attr) {
         // exactly the dynamic LDS it uses: a kernel with static LDS (a workgroup
@@ -373,6 +443,35 @@ static hipError_t launch_fast_sel(const FastArgs& a, hipStream_t s) {
             return launch_fast_t<T, L, MODE, 4, DK>(a, s);
     }
     return launch_fast_t<T, L, MODE, 1, DK>(a, s);
+}
+
+// instances per workgroup of the transposing pass of length L (as launched)
+template <typename T, int L>
+static uint32_t tout_ipw_t() {
+    if constexpr (L >= 64 && L <= 1024) {
+        const int m = fast_cols_mul();
+        if (m == 2)
+            return (uint32_t)fast_ipw<L, FAST_TOUT, T, 2>();
+        if (m == 4)
+            return (uint32_t)fast_ipw<L, FAST_TOUT, T, 4>();
+    }
+    return (uint32_t)fast_ipw<L, FAST_TOUT, T, 1>();
+}
+
+template <typename T>
+static uint32_t fast_tout_ipw_any(uint32_t L) {
+    switch (L) {
+    case 16: return tout_ipw_t<T, 16>();
+    case 32: return tout_ipw_t<T, 32>();
+    case 64: return tout_ipw_t<T, 64>();
+    case 128: return tout_ipw_t<T, 128>();
+    case 256: return tout_ipw_t<T, 256>();
+    case 512: return tout_ipw_t<T, 512>();
+    case 1024: return tout_ipw_t<T, 1024>();
+    case 2048: return tout_ipw_t<T, 2048>();
+    case 4096: return tout_ipw_t<T, 4096>();
+    default: return 0;
+    }
 }
 
 template <typename T, int MODE, int DK>

@@ -35,6 +35,9 @@ hipError_t launch_fft_f32(int maxe, const KfaFftArgs& a, uint32_t lds_bytes, hip
 // compile-time shaped power-of-two passes; mode: 0 row, 1 column, 2 row-in/column-out
 hipError_t launch_fast_f64(uint32_t L, int mode, const FastArgs& a, hipStream_t s);
 hipError_t launch_fast_f32(uint32_t L, int mode, const FastArgs& a, hipStream_t s);
+// instances per workgroup of the shaped transposing pass of length L (0: none)
+uint32_t fast_tout_ipw_f64(uint32_t L);
+uint32_t fast_tout_ipw_f32(uint32_t L);
 // fused on-chip two-pass transform of N*N points (N = 256)
 hipError_t launch_fused_f32(uint32_t N, const FusedArgs& f, uint32_t groups, hipStream_t s);
 hipError_t launch_fused_f64(uint32_t N, const FusedArgs& f, uint32_t groups, hipStream_t s);

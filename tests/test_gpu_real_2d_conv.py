@@ -57,6 +57,21 @@ def test_real_forward_many_per_tile(n):
     assert rel(got, ref) <= tol(n)
 
 
+@pytest.mark.parametrize("half", [False, True])
+@pytest.mark.parametrize("n,batch", [(1 << 15, 5), (1 << 16, 3), (1 << 17, 3), (1 << 19, 2)])
+def test_real_forward_two_pass_rebuild(n, batch, half):
+    """Power-of-two half lengths beyond one pass: the rebuild runs in the store of
+    the transposing second pass, whose workgroups hold rows k1 and N1 - k1 of the
+    same transform; several transforms in one launch, both builds."""
+    ft = np.float32 if half else np.float64
+    rng = np.random.default_rng(n + 17)
+    x = rng.uniform(-1, 1, (batch, n)).astype(ft)
+    got = K.ScalarPlan(n, half=half).forward(x)
+    ref = np.fft.rfft(x.astype(np.float64), axis=-1)
+    assert got.shape == ref.shape
+    assert rel(got, ref) <= (5e-7 * np.log2(n) if half else tol(n))
+
+
 @pytest.mark.parametrize("n", REAL_N)
 @pytest.mark.parametrize("norm", [True, False])
 def test_real_inverse(n, norm):
