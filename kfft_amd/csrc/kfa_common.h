@@ -73,7 +73,11 @@ struct KfaFftArgs {
     // b mod mid_rows == 0, at index b / mid_rows (Rader's A[0]).
     uint64_t mid_pw;
     uint64_t mid_dc;
-    uint32_t mid_rows, pad2;
+    uint32_t mid_rows;
+    // workgroup -> tile order (k_fft): 0 in dispatch order; n > 1: workgroups are
+    // dealt round-robin to n XCDs, and XCD x runs a contiguous range of tiles, so
+    // the tiles of one transform share that XCD's L2 (scattered Rader gathers)
+    uint32_t xcds;
     uint32_t L, nstages;
     uint32_t tpi, ipw;         // threads per instance, instances per workgroup
     uint32_t in_jfast;         // cooperative load: consecutive threads on consecutive instances

@@ -139,11 +139,14 @@ __device__ __forceinline__ void fast_tile(const FastArgs& A, uint64_t first, cx<
     // real-input rebuild (TOUT): slot j < IPW/2 of tile t of a transform holds
     // row a = t IPW/2 + j, slot j + IPW/2 its partner N1 - a (row N1/2 for a = 0),
     // so that every bin's partner bin h - k is in the workgroup (IPW divides N1)
-    const bool rpost = MODE == FAST_TOUT && (A.flags & KFA_FAST_RPOST) != 0;
+    // (compiled only where a tile can hold row pairs; the launcher refuses the rest)
+    constexpr bool can_rpost = MODE == FAST_TOUT && NST > 1 && IPW >= 2 && IPW % 2 == 0;
+    const bool rpost = can_rpost && (A.flags & KFA_FAST_RPOST) != 0;
     auto slot_inst = [&](uint32_t j) -> uint64_t {
         if (!rpost)
             return first + j;
-        const uint32_t n1 = A.out_C, half = IPW / 2;
+        constexpr uint32_t half = IPW / 2 > 0 ? IPW / 2 : 1;
+        const uint32_t n1 = A.out_C;
         const uint64_t b = first / n1;
         const uint32_t t = (uint32_t)(first - b * n1) / IPW;
         const uint32_t a = t * half + (j % half);
@@ -252,7 +255,7 @@ __device__ __forceinline__ void fast_tile(const FastArgs& A, uint64_t first, cx<
     }
     // ---- real-input rebuild: the rows' Z through LDS, then X[k] from Z[k] and
     // Z[h - k] (kfft_scalar.c:174-216), X[h] from Z[0]; h = N1 * L
-    if constexpr (MODE == FAST_TOUT && NST > 1) {
+    if constexpr (can_rpost) {
         if (rpost) {
             constexpr int RL = S::R(NST - 1), NB = E / RL, STR = L / RL;
             __syncthreads(); // every read of the last stage's operands is done

@@ -221,7 +221,14 @@ __device__ __forceinline__ void fft_tile(const KfaFftArgs& A, uint64_t first, cx
 template <typename T, int MAXE>
 __global__ void __launch_bounds__(MAXE <= 16 ? 512 : 256) k_fft(const KfaFftArgs A) {
     extern __shared__ __align__(16) unsigned char kfa_smem[];
-    fft_tile<T, MAXE>(A, (uint64_t)blockIdx.x * A.ipw, reinterpret_cast<cx<T>*>(kfa_smem));
+    uint32_t tile = blockIdx.x;
+    if (A.xcds > 1) {
+        // XCD x = b mod n takes tiles [x q + min(x, r), ...): q = G / n, r = G mod n
+        const uint32_t n = A.xcds, G = gridDim.x, q = G / n, r = G - q * n;
+        const uint32_t x = tile % n, j = tile / n;
+        tile = x * q + (x < r ? x : r) + j;
+    }
+    fft_tile<T, MAXE>(A, (uint64_t)tile * A.ipw, reinterpret_cast<cx<T>*>(kfa_smem));
 }
 
 template <typename T, int MAXE>

@@ -186,9 +186,19 @@ __device__ __forceinline__ bool dft16_mfma(cx<float>* v, const MfmaConst<SPLIT>&
     if constexpr (SPLIT == DFT_SPLIT_F16) {
         float mx = 0.0f;
         if (check) {
+            // max of |re|, |im| over the butterfly: one three-operand max with
+            // absolute-value modifiers per value (the data is finite, so the
+            // canonicalising moves fmaxf adds in IEEE mode are not needed), two
+            // independent chains
+            float m1 = 0.0f;
 #pragma unroll
-            for (int i = 0; i < 16; ++i)
-                mx = fmaxf(mx, fmaxf(fabsf(v[i].re), fabsf(v[i].im)));
+            for (int i = 0; i < 16; i += 2) {
+                asm("v_max3_f32 %0, |%1|, |%2|, %0" : "+v"(mx) : "v"(v[i].re), "v"(v[i].im));
+                asm("v_max3_f32 %0, |%1|, |%2|, %0"
+                    : "+v"(m1)
+                    : "v"(v[i + 1].re), "v"(v[i + 1].im));
+            }
+            mx = fmaxf(mx, m1);
             scaled = __any(!(mx >= 0.5f && mx < 16384.0f));
             narrow = !__any(!(mx >= 0.5f && mx <= 512.0f));
         }
