@@ -121,6 +121,8 @@ struct FastArgs {
     uint32_t pad; // KFA_FAST_TW4_HI: the four-step row length N2 (twiddle table stride)
     double scale;
     const void* rpost_st; // KFA_FAST_RPOST: st[i] = exp(-i pi ((i+1)/h + 1/2)), i < h
+    uint32_t xcds;        // > 1: XCD-contiguous tile order (as KfaFftArgs::xcds)
+    uint32_t pad1;
 };
 
 

@@ -454,6 +454,10 @@ hipError_t SubFft::run_four_fast(const KfaIo& load, const KfaIo& store, double s
     a.tw4 = tw4_full.p;
     a.flags = KFA_FAST_TW4 | ((load.flags & KFA_IO_CONJ1) ? KFA_FAST_CONJ_IN : 0) |
               ((nt & 1) ? KFA_FAST_NT_IN : 0) | ((nt & 4) ? KFA_FAST_NT_OUT : 0) | dft;
+    // FP64: XCD-contiguous tile order in both passes (measured: 2^20 x 16 and the
+    // real 2^18 x 512 forward 14% faster, 2^16 x 4096 4%; the FP32 passes are
+    // unchanged by it, and one-pass rows and 2D column passes slower)
+    a.xcds = prec == 8 ? xcd_tile_order() : 0u;
     hipError_t e = launch_fast(prec, N1, 1, a, s);
     if (e != hipSuccess)
         return e;
@@ -478,6 +482,7 @@ hipError_t SubFft::run_four_fast(const KfaIo& load, const KfaIo& store, double s
         b.flags |= KFA_FAST_RPOST;
         b.rpost_st = (const void*)store.pw;
     }
+    b.xcds = a.xcds;
     return launch_fast(prec, N2, 2, b, s);
 }
 

@@ -358,8 +358,13 @@ __device__ __forceinline__ void fast_tile(const FastArgs& A, uint64_t first, cx<
 template <typename T, int L, int MODE, int IPW, int DK>
 __global__ void __launch_bounds__(IPW* P2<L>::TPI) k_fast(const FastArgs A) {
     extern __shared__ __align__(16) unsigned char kfa_smem[];
-    fast_tile<T, L, MODE, IPW, DK>(A, (uint64_t)blockIdx.x * IPW,
-                                   reinterpret_cast<cx<T>*>(kfa_smem));
+    uint32_t tile = blockIdx.x;
+    if (A.xcds > 1) {
+        const uint32_t n = A.xcds, G = gridDim.x, q = G / n, r = G - q * n;
+        const uint32_t x = tile % n, j = tile / n;
+        tile = x * q + (x < r ? x : r) + j;
+    }
+    fast_tile<T, L, MODE, IPW, DK>(A, (uint64_t)tile * IPW, reinterpret_cast<cx<T>*>(kfa_smem));
 }
 
 template <int L, int MODE, typename T, int MUL>
