@@ -740,6 +740,27 @@ hipError_t SubFft::run(const KfaIo& load, const KfaIo& store, double scale, doub
     const bool simple = fast_io_ok(load, KFA_IO_CONJ1) &&
                         fast_io_ok(store, KFA_IO_CONJ2 | KFA_IO_SCALE) && scale_post == 1.0;
     const bool rec = recorder() != nullptr; // recording for the ragged executor: general passes
+    // both shaped passes over groups of transforms whose intermediate fits the
+    // chunk size (the second pass then reads part of it back from the caches)
+    auto four_fast_chunked = [&]() -> hipError_t {
+        const Tuning& tn = tuning();
+        uint64_t chunk = batch;
+        if (tn.chunk_bytes) {
+            chunk = tn.chunk_bytes / ((size_t)L * elem_bytes(prec));
+            chunk = std::max<uint64_t>(1, std::min<uint64_t>(chunk, batch));
+        }
+        const int64_t esz = (int64_t)elem_bytes(prec);
+        for (uint64_t c0 = 0; c0 < batch; c0 += chunk) {
+            const uint64_t nb = std::min<uint64_t>(chunk, batch - c0);
+            KfaIo ld = load, st = store;
+            ld.ptr += (uint64_t)((int64_t)c0 * load.dist_hi * esz);
+            st.ptr += (uint64_t)((int64_t)c0 * store.dist_hi * esz);
+            hipError_t e = run_four_fast(ld, st, scale, nb, scratch, tn.nt, s, dft);
+            if (e != hipSuccess)
+                return e;
+        }
+        return hipSuccess;
+    };
     if (four && (store.flags & KFA_IO_RPOST)) {
         // fused rebuild: the shaped two-pass form on plain contiguous rows only
         KfaIo st = store;
@@ -747,6 +768,7 @@ hipError_t SubFft::run(const KfaIo& load, const KfaIo& store, double scale, doub
         if (rec || !rpost_two_pass() || load.C != 1 || store.C != 1 || scale_post != 1.0 ||
             !fast_io_ok(load, 0) || !fast_io_ok(st, 0))
             return hipErrorInvalidValue;
+        // whole batch: chunks measured slower here (2^18 x 512: 1.16 against 1.12 ms)
         return run_four_fast(load, store, scale, batch, scratch, tuning().nt, s, dft);
     }
     if (!four) {
@@ -782,25 +804,8 @@ hipError_t SubFft::run(const KfaIo& load, const KfaIo& store, double scale, doub
     if (onchip && !rec && !dft && simple && tw4_full.p && N1 == 256 && N2 == 256 && s == default_stream() &&
         load.stride == 1 && store.stride == 1 && fused_queues() > 0)
         return run_fused(load, store, scale, batch, s);
-    if (simple && tw4_full.p && !rec) {
-        const Tuning& tn = tuning();
-        uint64_t chunk = batch;
-        if (tn.chunk_bytes) {
-            chunk = tn.chunk_bytes / ((size_t)L * elem_bytes(prec));
-            chunk = std::max<uint64_t>(1, std::min<uint64_t>(chunk, batch));
-        }
-        const int64_t esz = (int64_t)elem_bytes(prec);
-        for (uint64_t c0 = 0; c0 < batch; c0 += chunk) {
-            const uint64_t nb = std::min<uint64_t>(chunk, batch - c0);
-            KfaIo ld = load, st = store;
-            ld.ptr += (uint64_t)((int64_t)c0 * load.dist_hi * esz);
-            st.ptr += (uint64_t)((int64_t)c0 * store.dist_hi * esz);
-            hipError_t e = run_four_fast(ld, st, scale, nb, scratch, tn.nt, s, dft);
-            if (e != hipSuccess)
-                return e;
-        }
-        return hipSuccess;
-    }
+    if (simple && tw4_full.p && !rec)
+        return four_fast_chunked();
     // pass A: columns of length N1 (stride N2), twiddle W_L^(n2 k1) on store
     KfaIo ia = load;
     ia.C = N2;
