@@ -26,9 +26,20 @@
 
 namespace kfa {
 
-template <typename T, int R, int MAXE>
+// IO of a stage: LDS to LDS, or (the last stage of a plain row layout) LDS to
+// memory.  "Plain": element e at base + (lgb + e e_mul) stride, complex or real,
+// with at most conjugation and scaling (fused_io_plain).  (Reading stage 0's
+// inputs from memory the same way spilled registers of the pass.)
+enum { ST_LDS = 0, ST_OUT = 2 };
+
+__device__ __forceinline__ bool fused_io_plain(const KfaIo& io, uint32_t extra) {
+    return (io.flags & ~(KFA_IO_REAL | KFA_IO_CONJ1 | KFA_IO_CONJ2 | KFA_IO_INST | extra)) == 0 &&
+           io.valid == 0xffffffffu;
+}
+
+template <typename T, int R, int MAXE, int IO = ST_LDS>
 __device__ __forceinline__ void lds_stage(const KfaFftArgs& A, uint32_t s, cx<T>* my, uint32_t lane,
-                                          bool act, const cx<T>* ltw) {
+                                          bool act, const cx<T>* ltw, uint64_t first = 0) {
     constexpr int NB = (MAXE / R) > 0 ? (MAXE / R) : 1;
     const uint32_t nb = A.L / R, ns = A.st[s].ns, tpi = A.tpi, magic = A.st[s].ns_magic;
     // b mod ns with b, ns < 2^16 (the LDS pass lengths): one multiply-high
@@ -85,18 +96,53 @@ __device__ __forceinline__ void lds_stage(const KfaFftArgs& A, uint32_t s, cx<T>
             }
             dft<T, R>(v[i]);
             const uint32_t base = (b - k) * R + k;
+            if constexpr (IO == ST_OUT) {
+                const KfaIo& io = A.out;
+                const Ctx cio = make_ctx(io, first + threadIdx.x / tpi);
+                const int64_t es = (int64_t)io.e_mul * io.stride;
+                const int64_t a0 = cio.base + (int64_t)cio.lgb * io.stride;
+                const bool cj = ((io.flags & KFA_IO_CONJ1) != 0) != ((io.flags & KFA_IO_CONJ2) != 0);
+                if (io.flags & KFA_IO_SCALE) {
+                    const T sc = (T)A.scale;
 #pragma unroll
-            for (int r = 0; r < R; ++r)
-                my[lds_index<T>(base + r * ns)] = v[i][r];
+                    for (int r = 0; r < R; ++r) {
+                        v[i][r].re *= sc;
+                        v[i][r].im *= sc;
+                    }
+                }
+                if (io.flags & KFA_IO_REAL) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        ((T*)io.ptr)[a0 + (int64_t)(base + r * ns) * es] = v[i][r].re;
+                } else {
+                    if (cj) {
+#pragma unroll
+                        for (int r = 0; r < R; ++r)
+                            v[i][r].im = -v[i][r].im;
+                    }
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        ((cx<T>*)io.ptr)[a0 + (int64_t)(base + r * ns) * es] = v[i][r];
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    my[lds_index<T>(base + r * ns)] = v[i][r];
+            }
         }
     }
-    __syncthreads();
+    if constexpr (IO != ST_OUT)
+        __syncthreads();
 }
 
 #define KFA_STAGE_CASE(R)                                                                         \
     case R:                                                                                       \
-        if constexpr (R <= MAXE)                                                                  \
-            lds_stage<T, R, MAXE>(A, s, my, lane, act, ltw);                                      \
+        if constexpr (R <= MAXE) {                                                                \
+            if (sio == ST_OUT)                                                                    \
+                lds_stage<T, R, MAXE, ST_OUT>(A, s, my, lane, act, ltw, first);                   \
+            else                                                                                  \
+                lds_stage<T, R, MAXE>(A, s, my, lane, act, ltw);                                  \
+        }                                                                                         \
         break;
 
 // One workgroup tile of a pass: instances [first, first + ipw).  The
@@ -113,6 +159,10 @@ __device__ __forceinline__ void fft_tile(const KfaFftArgs& A, uint64_t first, cx
     // on consecutive threads (rows), or instances on consecutive threads (in_jfast,
     // columns); each thread walks elements lane, lane + tpi, ...
     const uint32_t tpi = A.tpi, ipw = A.ipw;
+    // plain row layouts: the last stage writes to memory directly (one LDS sweep
+    // and one barrier fewer); column layouts keep the cooperative store, whose
+    // consecutive threads walk instances
+    const bool fuse_out = !A.out_jfast && A.nstages >= 2 && fused_io_plain(A.out, KFA_IO_SCALE);
     {
         const uint32_t j = A.in_jfast ? tid % ipw : tid / tpi;
         const uint32_t ln = A.in_jfast ? tid / ipw : tid % tpi;
@@ -177,6 +227,8 @@ __device__ __forceinline__ void fft_tile(const KfaFftArgs& A, uint64_t first, cx
             __syncthreads();
         }
         for (uint32_t s = 0; s < A.nstages; ++s) {
+            const int sio =
+                (fuse_out && round + 1 == rounds && s + 1 == A.nstages) ? ST_OUT : ST_LDS;
             switch (A.st[s].radix) {
                 KFA_STAGE_CASE(2)
                 KFA_STAGE_CASE(3)
@@ -200,7 +252,7 @@ __device__ __forceinline__ void fft_tile(const KfaFftArgs& A, uint64_t first, cx
     }
 
     // 3. store, thread mapping as the load's (out_jfast)
-    {
+    if (!fuse_out) {
         const bool addx0 = (A.out.flags & KFA_IO_ADDX0) != 0;
         const uint32_t j = A.out_jfast ? tid % ipw : tid / tpi;
         const uint32_t ln = A.out_jfast ? tid / ipw : tid % tpi;
