@@ -2,6 +2,7 @@
 #include "kfa_engine.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -621,8 +622,9 @@ uint32_t num_cus() {
 uint32_t fused_queues() { return env_u32("KFFT_AMD_FUSED", 1) ? xcd_queues() : 0u; }
 
 // XCD count when the probe saw workgroup b of an ordinary grid on XCD b mod n
-// (round-robin dispatch), else 0
-static uint32_t g_xcd_rr = 0;
+// (round-robin dispatch), else 0 (atomic: plans may run on threads other than
+// the one whose plan build ran the probe)
+static std::atomic<uint32_t> g_xcd_rr{0};
 
 uint32_t xcd_queues() {
     static std::once_flag once;
@@ -659,7 +661,7 @@ uint32_t xcd_queues() {
             bool rr = true;
             for (uint32_t b = 0; b < groups; ++b)
                 rr = rr && h[b] == b % nq;
-            g_xcd_rr = rr ? nq : 0u;
+            g_xcd_rr.store(rr ? nq : 0u, std::memory_order_relaxed);
         }
     });
     return nq;
@@ -672,7 +674,7 @@ static uint32_t xcd_tile_order() {
         const char* e = getenv("KFFT_AMD_XCD_TILES");
         return !e || atoi(e) != 0;
     }();
-    return on ? g_xcd_rr : 0u;
+    return on ? g_xcd_rr.load(std::memory_order_relaxed) : 0u;
 }
 
 hipError_t SubFft::run_fused(const KfaIo& load, const KfaIo& store, double scale, uint64_t batch,
